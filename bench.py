@@ -1,0 +1,320 @@
+"""Sparse gemv benchmark on one MI355X.
+
+Headline: plain CSR SpMV y = A x, FP32 values, int32 indices, A 16M x 16M
+with row lengths ~ Zipf(1.5) clipped to [1, 4096], sorted distinct columns
+(stratified uniform), vals N(0,1), x U[-1,1] -- inputs generated on the GPU
+and resident in HBM before timing. One step = one drx_spmv over the whole
+matrix. value = algorithmic GB/s (the reference's byte formula,
+examples/sp/gemv_benchmark.cpp:135-138) over the timed steps.
+
+Also reported in the same JSON line (`forms`): the 2-D Laplacian FP64 form
+(1e6 rows, four rotating copies so the 256 MB MALL cannot hold them), the
+banded forward / adjoint / dvals / fused-residual forms (4M rows, band +-16),
+the low-precision storage forms (3-D Laplacian 256^3 and the Zipf matrix in
+BF16 / FP16 vs FP32 / FP64) with their error against the FP64 form, and the
+batched form.
+
+cpu_baseline: the reference's CPU path (mp-bench GemvEq_DR without --sycl:
+csr_eq shares, one single-threaded rank per core, sequential local loop,
+Gatherv + boundary-row assembly) as restated in oracle/, run on a bounded
+sample of the headline matrix (its first rows), on rank 0 only.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "distributed-ranges_amd"))
+
+import drx  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+SZ = {torch.float64: 8, torch.float32: 4, torch.bfloat16: 2, torch.float16: 2,
+      torch.int32: 4, torch.int64: 8}
+
+
+def spmv_bytes(m, n, nnz, sv, si, sx, sy, k=1):
+    """B = nnz*(sv+si) + si*(m+1) + k*(sx*n + sy*m)."""
+    return nnz * (sv + si) + si * (m + 1) + k * (sx * n + sy * m)
+
+
+def time_launches(fn, steps, warmup, stream=None):
+    """(wall ms per step over `steps` back-to-back launches, mean event ms per
+    launch). Events are recorded on the stream the kernels run on."""
+    stream = stream or torch.cuda.current_stream()
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / steps
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    for a, b in ev:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    ev_ms = sum(a.elapsed_time(b) for a, b in ev) / steps
+    return wall_ms, ev_ms
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# Headline
+# ---------------------------------------------------------------------------
+def headline(steps, warmup):
+    n = 16_000_000
+    t0 = time.perf_counter()
+    A = drx.gen.zipf_csr(n, n, s=1.5, lmax=4096, seed=0, vdtype=torch.float32,
+                         itype=torch.int32)
+    x = drx.gen.uniform(n, torch.float32, -1.0, 1.0, seed=1)
+    y = torch.empty(n, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    log("zipf matrix: %d rows, %d nnz (mean %.2f/row), generated in %.1f s"
+        % (n, A.nnz, A.nnz / n, time.perf_counter() - t0))
+    B = spmv_bytes(n, n, A.nnz, 4, 4, 4, 4)
+    wall_ms, ev_ms = time_launches(lambda: A.matvec(x, y=y), steps, warmup)
+    log("zipf fp32: %.3f ms/step wall, %.3f ms/launch (events)" % (wall_ms, ev_ms))
+    return A, x, y, B, wall_ms, ev_ms
+
+
+def cpu_baseline(A, x, y_gpu, target_s=10.0):
+    """Restated GemvEq_DR CPU path on the first rows of the headline matrix."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import _oracle  # the CPU restatement (test infrastructure)
+
+    orc = _oracle.Oracle()
+    n = A.shape[1]
+    rp_all = A.rowptr
+    # sample: first R rows, about 1/8 of the matrix
+    R = A.shape[0] // 8
+    nnz_s = int(rp_all[R].item())
+    rp = rp_all[:R + 1].cpu().numpy()
+    ci = A.colind[:nnz_s].cpu().numpy()
+    v = A.vals[:nnz_s].cpu().numpy()
+    xs = x.cpu().numpy()
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    P = max(1, min(16, cores))
+    # one untimed probe to size the repetitions
+    _, t1, _ = orc.gemveq_cpu_bench(rp, ci, v, xs, (R, n), P, warmup=0, reps=1)
+    reps = int(max(3, min(50, target_s / max(t1, 1e-3))))
+    res, med, _ = orc.gemveq_cpu_bench(rp, ci, v, xs, (R, n), P, warmup=1, reps=reps)
+    B_s = spmv_bytes(R, n, nnz_s, 4, 4, 4, 4)
+    yg = y_gpu[:R].double().cpu().numpy()
+    yc = res.astype(np.float64)
+    rel = float(np.linalg.norm(yg - yc) / max(np.linalg.norm(yc), 1e-300))
+    return {
+        "value": round(B_s / med / 1e9, 3), "unit": "GB/s", "cores": P, "kind": "port",
+        "sample": ("first %d rows (%d nnz, 1/8) of the 16M-row Zipf FP32 matrix, full x; "
+                   "median of %d GemvEq_DR calls (csr_eq shares over %d single-threaded "
+                   "std::thread ranks, Gatherv + boundary-row assembly), %.4f s/call"
+                   % (R, nnz_s, reps, P, med)),
+        "ms_per_call": round(med * 1e3, 3),
+        "gpu_vs_reference_cpu_rel_err_norm2": rel,
+    }
+
+
+# ---------------------------------------------------------------------------
+# Forms
+# ---------------------------------------------------------------------------
+def rel_err(y, y64):
+    y = y.double()
+    return float((y - y64).norm() / y64.norm())
+
+
+def form_laplacian2d(steps):
+    """2-D 5-point Laplacian 1000 x 1000, FP64, int32: 80 MB per copy, four
+    copies rotated so every launch streams from HBM."""
+    mats = [drx.gen.laplacian(2, 1000, vdtype=torch.float64) for _ in range(4)]
+    xs = [drx.gen.uniform(1_000_000, torch.float64, -1, 1, seed=20261202 + i) for i in range(4)]
+    ys = [torch.empty(1_000_000, dtype=torch.float64, device="cuda") for _ in range(4)]
+    it = [0]
+
+    def rot():
+        i = it[0] % 4
+        it[0] += 1
+        mats[i].matvec(xs[i], y=ys[i])
+
+    B = spmv_bytes(1_000_000, 1_000_000, mats[0].nnz, 8, 4, 8, 8)
+    _, ev_ms = time_launches(rot, steps * 4, 8)
+    _, hot_ms = time_launches(lambda: mats[0].matvec(xs[0], y=ys[0]), steps, 5)
+    return {"rows": 1_000_000, "nnz": mats[0].nnz, "dtype": "f64",
+            "GBps": round(B / ev_ms / 1e6, 1), "frac": round(B / ev_ms / 1e6 / HBM_PEAK_GBS, 4),
+            "us_per_launch": round(ev_ms * 1e3, 2),
+            "GBps_hot_cache": round(B / hot_ms / 1e6, 1)}
+
+
+def form_band_adjoint(steps):
+    """Band +-16 (33 diagonals), 4M rows, FP32: forward, adjoint (CSC with
+    permuted vals), dvals and the fused forward+residual+dvals pass."""
+    m = 4_000_000
+    A = drx.gen.band(m, 16, 16, vdtype=torch.float32, seed=31)
+    x = drx.gen.uniform(m, torch.float32, -1, 1, seed=32)
+    t = drx.gen.uniform(m, torch.float32, -1, 1, seed=33)
+    y = torch.empty(m, dtype=torch.float32, device="cuda")
+    dx = torch.empty(m, dtype=torch.float32, device="cuda")
+    dv = torch.empty(A.nnz, dtype=torch.float32, device="cuda")
+    r = torch.empty(m, dtype=torch.float32, device="cuda")
+    A.transpose()
+    nnz = A.nnz
+    fwd_B = spmv_bytes(m, m, nnz, 4, 4, 4, 4)
+    adj_B = spmv_bytes(m, m, nnz, 4, 4, 4, 4) + 4 * nnz  # + permutation
+    dv_B = 4 * nnz + 4 * (m + 1) + 4 * nnz + 4 * m + 4 * m
+    fu_B = fwd_B + 4 * m + 4 * m + 4 * nnz  # + t read, r write, dvals write
+    _, f_ms = time_launches(lambda: A.matvec(x, y=y), steps, 3)
+    _, a_ms = time_launches(lambda: A.rmatvec(y, dx=dx), steps, 3)
+    _, d_ms = time_launches(lambda: A.dvals(x, y, out=dv), steps, 3)
+    _, u_ms = time_launches(lambda: A.fused_residual(x, t, y=y, r=r, dvals=dv), steps, 3)
+    A64 = drx.CsrMatrix(A.rowptr, A.colind, A.vals.double(), A.shape)
+    y64 = A64.matvec(x.double())
+    e_fwd = rel_err(A.matvec(x), y64)
+    e_adj = rel_err(A.rmatvec(y64.float()), A64.rmatvec(y64))
+    out = {"rows": m, "nnz": nnz, "dtype": "f32"}
+    for name, B, ms, err in (("forward", fwd_B, f_ms, e_fwd), ("adjoint", adj_B, a_ms, e_adj),
+                             ("dvals", dv_B, d_ms, None), ("fused_fwd_residual_dvals", fu_B, u_ms, None)):
+        out[name] = {"GBps": round(B / ms / 1e6, 1), "frac": round(B / ms / 1e6 / HBM_PEAK_GBS, 4),
+                     "us_per_launch": round(ms * 1e3, 1)}
+        if err is not None:
+            out[name]["rel_err_vs_fp64"] = err
+    sep = f_ms + d_ms
+    out["fused_speedup_vs_separate_fwd_plus_dvals"] = round(sep / u_ms, 3)
+    return out
+
+
+def form_lowprec(steps, zipf=None):
+    """3-D 7-point Laplacian 256^3 (16.7M rows, 117M nnz) with vals and x
+    stored as FP64 / FP32 / BF16 / FP16 (FP32 accumulation for the 16-bit
+    storage), plus the Zipf matrix in BF16; error vs the FP64 form."""
+    g = 256
+    m = g ** 3
+    A64 = drx.gen.laplacian(3, g, vdtype=torch.float64)
+    x64 = drx.gen.uniform(m, torch.float64, -1, 1, seed=41)
+    y64 = A64.matvec(x64)
+    out = {"rows": m, "nnz": A64.nnz}
+    base_ms = {}
+    for dt, name in ((torch.float64, "f64"), (torch.float32, "f32"),
+                     (torch.bfloat16, "bf16"), (torch.float16, "f16")):
+        vals = A64.vals.to(dt)
+        x = x64.to(dt)
+        ydt = torch.float64 if dt == torch.float64 else torch.float32
+        y = torch.empty(m, dtype=ydt, device="cuda")
+        _, ms = time_launches(lambda: A64.matvec(x, y=y, vals=vals), steps, 3)
+        B = spmv_bytes(m, m, A64.nnz, SZ[dt], 4, SZ[dt], SZ[ydt])
+        base_ms[name] = ms
+        out[name] = {"GBps": round(B / ms / 1e6, 1), "frac": round(B / ms / 1e6 / HBM_PEAK_GBS, 4),
+                     "us_per_launch": round(ms * 1e3, 1),
+                     "rel_err_vs_fp64": rel_err(A64.matvec(x, vals=vals), y64)}
+    for name in ("bf16", "f16"):
+        out[name]["speedup_vs_f32"] = round(base_ms["f32"] / base_ms[name], 3)
+        out[name]["speedup_vs_f64"] = round(base_ms["f64"] / base_ms[name], 3)
+    del A64
+    if zipf is not None:
+        A, x = zipf
+        vb = A.vals.to(torch.bfloat16)
+        xb = x.to(torch.bfloat16)
+        y = torch.empty(A.shape[0], dtype=torch.float32, device="cuda")
+        _, ms = time_launches(lambda: A.matvec(xb, y=y, vals=vb), steps, 3)
+        _, ms32 = time_launches(lambda: A.matvec(x, y=y), steps, 3)
+        B = spmv_bytes(A.shape[0], A.shape[1], A.nnz, 2, 4, 2, 4)
+        y32 = A.matvec(x)
+        vd = A.vals.double()
+        A64z = drx.CsrMatrix(A.rowptr, A.colind, vd, A.shape)
+        yz64 = A64z.matvec(x.double())
+        del A64z, vd
+        out["zipf_bf16"] = {"GBps": round(B / ms / 1e6, 1), "frac": round(B / ms / 1e6 / HBM_PEAK_GBS, 4),
+                            "us_per_launch": round(ms * 1e3, 1),
+                            "speedup_vs_f32": round(ms32 / ms, 3),
+                            "rel_err_vs_fp64": rel_err(A.matvec(xb, vals=vb), yz64),
+                            "f32_rel_err_vs_fp64": rel_err(y32, yz64)}
+    return out
+
+
+# ---------------------------------------------------------------------------
+def read_traffic():
+    """HBM bytes per launch of the headline kernel from the committed
+    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
+    path = os.path.join(REPO, "profiles", "pmc_spmv_zipf16m_f32.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return int(float(line.split(":", 1)[1].strip()))
+            except ValueError:
+                return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--forms", default="all", help="all | none | comma list")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build runs the kernel on one GPU's "
+              "share only (see the sharding notes); --gpus %d is not supported."
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    torch.cuda.set_device(0)
+    A, x, y, B, wall_ms, ev_ms = headline(args.steps, args.warmup)
+    value = B / (wall_ms * 1e-3) / 1e9
+    achieved = B / (ev_ms * 1e-3) / 1e9
+    result = {
+        "metric": "SpMV GB/s (% of 8 TB/s) fwd/adjoint, batched & bf16 vs plain, at stated rel error",
+        "value": round(value, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall_ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "plain CSR SpMV y=A*x, FP32 vals, int32 indices, 16M x 16M, "
+                               "nnz/row Zipf(s=1.5) clipped to [1,4096], sorted distinct "
+                               "stratified-uniform columns, vals N(0,1), x U[-1,1]",
+                   "rows": A.shape[0], "cols": A.shape[1], "nnz": A.nnz,
+                   "mean_nnz_per_row": round(A.nnz / A.shape[0], 3),
+                   "algorithmic_bytes_per_step": B, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": read_traffic(), "kernel": "drx::k_spmv_fwd<2048,f32,i32>",
+                     "ms_per_launch": round(ev_ms, 4)},
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        result["cpu_baseline"] = cpu_baseline(A, x, y, args.cpu_seconds)
+    forms = {}
+    want = set(["laplacian2d_f64", "band_adjoint", "lowprec"]) if args.forms == "all" else \
+        set() if args.forms == "none" else set(args.forms.split(","))
+    steps_f = max(5, min(args.steps, 20))
+    if "lowprec" in want:
+        log("form lowprec ...")
+        forms["lowprec"] = form_lowprec(steps_f, zipf=(A, x))
+    del A
+    torch.cuda.empty_cache()
+    if "laplacian2d_f64" in want:
+        log("form laplacian2d_f64 ...")
+        forms["laplacian2d_f64"] = form_laplacian2d(steps_f)
+    if "band_adjoint" in want:
+        log("form band_adjoint ...")
+        forms["band_adjoint"] = form_band_adjoint(steps_f)
+    if forms:
+        result["forms"] = forms
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,169 @@
+// drx_common.hpp -- shared device helpers for the gfx950 sparse gemv kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "drx_spmv.h"
+
+namespace drx {
+
+// ---------------------------------------------------------------------------
+// Errors: every C entry point returns a status and never throws; the message
+// of the last failure on the calling thread is kept for drx_last_error().
+// ---------------------------------------------------------------------------
+struct Error : std::runtime_error {
+  int status;
+  Error(int s, const std::string &m) : std::runtime_error(m), status(s) {}
+};
+
+void set_last_error(const char *msg);
+
+#define DRX_REQUIRE(cond, msg)                                                 \
+  do {                                                                         \
+    if (!(cond)) throw ::drx::Error(DRX_ERR_INVALID, msg);                     \
+  } while (0)
+
+#define DRX_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      throw ::drx::Error(e_ == hipErrorOutOfMemory ? DRX_ERR_NOMEM             \
+                                                   : DRX_ERR_HIP,              \
+                         std::string(#call) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+template <typename F> int guarded(F &&f) {
+  try {
+    f();
+    return DRX_OK;
+  } catch (const Error &e) {
+    set_last_error(e.what());
+    return e.status;
+  } catch (const std::bad_alloc &) {
+    set_last_error("host allocation failed");
+    return DRX_ERR_NOMEM;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return DRX_ERR_INVALID;
+  } catch (...) {
+    set_last_error("unknown error");
+    return DRX_ERR_INVALID;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Storage types
+// ---------------------------------------------------------------------------
+struct bf16 {
+  uint16_t bits;
+};
+using f16 = _Float16;
+
+__device__ __forceinline__ float to_f32(bf16 v) {
+  return __uint_as_float(static_cast<uint32_t>(v.bits) << 16);
+}
+__device__ __forceinline__ float to_f32(f16 v) { return static_cast<float>(v); }
+__device__ __forceinline__ float to_f32(float v) { return v; }
+
+template <typename TA, typename T>
+__device__ __forceinline__ TA to_acc(T v) {
+  if constexpr (std::is_same_v<T, bf16> || std::is_same_v<T, f16>)
+    return static_cast<TA>(to_f32(v));
+  else
+    return static_cast<TA>(v);
+}
+
+// Round-to-nearest-even float -> bf16 (NaN kept quiet).
+__device__ __forceinline__ bf16 f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu))
+    return bf16{static_cast<uint16_t>((u >> 16) | 0x40)};
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return bf16{static_cast<uint16_t>(u >> 16)};
+}
+
+template <typename TY, typename TA>
+__device__ __forceinline__ TY from_acc(TA v) {
+  if constexpr (std::is_same_v<TY, bf16>)
+    return f32_to_bf16(static_cast<float>(v));
+  else if constexpr (std::is_same_v<TY, f16>)
+    return static_cast<f16>(static_cast<float>(v));
+  else
+    return static_cast<TY>(v);
+}
+
+// Streaming (read-once) loads of the matrix arrays: non-temporal, so that the
+// L2 keeps the gathered x instead of vals/colind.
+template <typename T> __device__ __forceinline__ T ld_stream(const T *p) {
+  if constexpr (sizeof(T) == 4 || sizeof(T) == 8)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+
+// ---------------------------------------------------------------------------
+// Lane-group sums with DPP (within 16-lane rows) and cross-row shuffles.
+// G lanes, aligned to G; every lane of the group ends with the group sum.
+// ---------------------------------------------------------------------------
+template <int CTRL> __device__ __forceinline__ float dpp_f32(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL> __device__ __forceinline__ double dpp_f64(double v) {
+  int64_t b = __double_as_longlong(v);
+  int lo = static_cast<int>(b & 0xffffffff);
+  int hi = static_cast<int>(b >> 32);
+  lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xf, 0xf, false);
+  hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xf, 0xf, false);
+  return __longlong_as_double((static_cast<int64_t>(hi) << 32) |
+                              static_cast<uint32_t>(lo));
+}
+template <int CTRL, typename T> __device__ __forceinline__ T dpp(T v) {
+  if constexpr (std::is_same_v<T, double>)
+    return dpp_f64<CTRL>(v);
+  else
+    return dpp_f32<CTRL>(v);
+}
+
+// DPP controls (GFX9 encoding).
+constexpr int kQuadXor1 = 0xB1;      // quad_perm [1,0,3,2]
+constexpr int kQuadXor2 = 0x4E;      // quad_perm [2,3,0,1]
+constexpr int kRowHalfMirror = 0x141;  // lane i <- 7-i within 8
+constexpr int kRowMirror = 0x140;      // lane i <- 15-i within 16
+
+template <int G, typename T> __device__ __forceinline__ T group_sum(T v) {
+  if constexpr (G >= 2) v += dpp<kQuadXor1>(v);
+  if constexpr (G >= 4) v += dpp<kQuadXor2>(v);
+  if constexpr (G >= 8) v += dpp<kRowHalfMirror>(v);
+  if constexpr (G >= 16) v += dpp<kRowMirror>(v);
+  if constexpr (G >= 32) v += __shfl_xor(v, 16, 64);
+  if constexpr (G >= 64) v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Row spans of a CSR share. rowptr holds absolute offsets; base is the
+// share's first nnz; the row's span is clamped into [0, nnz). Row `nrows` is
+// the end: rowptr[nrows] with a sentinel, else nnz.
+// ---------------------------------------------------------------------------
+template <typename TI>
+__device__ __forceinline__ int64_t row_begin(const TI *rowptr, int64_t base,
+                                             int64_t nrows, int64_t nnz,
+                                             int has_sentinel, int64_t r) {
+  if (r < nrows || has_sentinel) {
+    int64_t v = static_cast<int64_t>(rowptr[r]) - base;
+    return v < 0 ? 0 : (v > nnz ? nnz : v);
+  }
+  return nnz;
+}
+
+// Number of XCDs that a launch's workgroups are dealt over round-robin.
+constexpr int kXcds = 8;
+
+}  // namespace drx

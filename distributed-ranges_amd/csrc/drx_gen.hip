@@ -1,0 +1,529 @@
+// drx_gen.hip -- device-side synthetic inputs (counter-based, reproducible on
+// any box), dtype conversion, and the tuning knobs of the library.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <atomic>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "drx_common.hpp"
+#include "drx_internal.hpp"
+
+namespace drx {
+
+// ---------------------------------------------------------------------------
+// Tuning knobs (process-wide; 0 = default). The environment variable
+// DRX_BLOCK_NNZ seeds block_nnz when nothing was set through the API.
+// ---------------------------------------------------------------------------
+static std::atomic<int64_t> g_block_nnz{0};
+
+int64_t tuned_block_nnz() {
+  int64_t v = g_block_nnz.load();
+  if (v > 0) return v;
+  if (const char *e = std::getenv("DRX_BLOCK_NNZ")) {
+    long long w = std::atoll(e);
+    if (w >= 64 && w <= 8192 && (w & (w - 1)) == 0) return w;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Counter-based randomness: splitmix64 finalizer.
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+// Stream `seed`, counter i.
+__host__ __device__ __forceinline__ uint64_t rnd(uint64_t seed, uint64_t i) {
+  return mix64(mix64(seed) ^ (i * 0xD1B54A32D192ED03ull));
+}
+__host__ __device__ __forceinline__ double u01(uint64_t h) {
+  return static_cast<double>(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+template <typename T> __device__ __forceinline__ T store_as(double v) {
+  if constexpr (std::is_same_v<T, bf16>)
+    return f32_to_bf16(static_cast<float>(v));
+  else if constexpr (std::is_same_v<T, f16>)
+    return static_cast<f16>(static_cast<float>(v));
+  else
+    return static_cast<T>(v);
+}
+
+template <typename T>
+__global__ void k_uniform(T *out, int64_t n, double lo, double hi,
+                          uint64_t seed) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       i < n; i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    out[i] = store_as<T>(lo + (hi - lo) * u01(rnd(seed, i)));
+}
+
+template <typename T>
+__global__ void k_normal(T *out, int64_t n, uint64_t seed) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       i < n; i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const double u1 = (static_cast<double>(rnd(seed, 2 * i) >> 11) + 1.0) *
+                      (1.0 / 9007199254740992.0);
+    const double u2 = u01(rnd(seed, 2 * i + 1));
+    out[i] = store_as<T>(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+  }
+}
+
+static dim3 grid1d(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 65536 * 8) b = 65536 * 8;
+  if (b < 1) b = 1;
+  return dim3(static_cast<unsigned>(b));
+}
+
+// ---------------------------------------------------------------------------
+// Row pointers from row lengths: len[1..n] in a 64-bit buffer, inclusive
+// scan, then narrowed to the index type.
+// ---------------------------------------------------------------------------
+template <typename TI>
+__global__ void k_narrow(const int64_t *src, TI *dst, int64_t n) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       i < n; i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    dst[i] = static_cast<TI>(src[i]);
+}
+
+// rp64[0..nrows] holds 0 followed by the lengths; scans it in place and
+// writes the index-typed rowptr. Returns the total (synchronizes).
+static int64_t finish_rowptr(int64_t *rp64, int64_t nrows, int itype,
+                             void *rowptr, hipStream_t s) {
+  size_t temp_bytes = 0;
+  const int64_t n = nrows + 1;
+  DRX_REQUIRE(n <= INT32_MAX, "too many rows for the scan");
+  DRX_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, temp_bytes, rp64, rp64,
+                                           static_cast<int>(n), s));
+  void *temp = nullptr;
+  DRX_HIP(hipMalloc(&temp, std::max<size_t>(temp_bytes, 1)));
+  DRX_HIP(hipcub::DeviceScan::InclusiveSum(temp, temp_bytes, rp64, rp64,
+                                           static_cast<int>(n), s));
+  int64_t total = 0;
+  DRX_HIP(hipMemcpyAsync(&total, rp64 + nrows, sizeof(int64_t),
+                         hipMemcpyDeviceToHost, s));
+  DRX_HIP(hipStreamSynchronize(s));
+  (void)hipFree(temp);
+  if (itype == DRX_I32) {
+    DRX_REQUIRE(total <= INT32_MAX, "nnz exceeds the 32-bit index type");
+    hipLaunchKernelGGL(k_narrow<int32_t>, grid1d(n), dim3(256), 0, s, rp64,
+                       static_cast<int32_t *>(rowptr), n);
+  } else {
+    DRX_HIP(hipMemcpyAsync(rowptr, rp64, n * sizeof(int64_t),
+                           hipMemcpyDeviceToDevice, s));
+  }
+  DRX_HIP(hipGetLastError());
+  DRX_HIP(hipStreamSynchronize(s));
+  return total;
+}
+
+// ---------------------------------------------------------------------------
+// Zipf row lengths: P(L = l) = l^-s / zeta(s) for l < lmax, the remaining
+// mass at lmax (a Zipf draw clipped to [1, lmax]).
+// ---------------------------------------------------------------------------
+__global__ void k_zipf_len(int64_t *rp64, int64_t nrows, const double *cdf,
+                           int64_t lmax, uint64_t seed) {
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       r < nrows; r += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const double u = u01(rnd(seed, r));
+    int64_t lo = 0, hi = lmax - 1;  // first index with cdf[idx] > u
+    while (lo < hi) {
+      int64_t mid = (lo + hi) / 2;
+      if (cdf[mid] > u)
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    rp64[r + 1] = lo + 1;
+    if (r == 0) rp64[0] = 0;
+  }
+}
+
+static double zeta(double s) {
+  const int N = 100000;
+  double acc = 0;
+  for (int i = N - 1; i >= 1; i--) acc += std::pow(static_cast<double>(i), -s);
+  const double n = N;
+  acc += std::pow(n, 1 - s) / (s - 1) + 0.5 * std::pow(n, -s) +
+         s * std::pow(n, -s - 1) / 12.0;
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
+// Stratified sorted distinct columns: entry i of a row of length L takes
+// column floor(i*n/L) + (hash % (floor((i+1)*n/L) - floor(i*n/L))).
+// One wavefront per row.
+// ---------------------------------------------------------------------------
+template <typename TI>
+__global__ void k_stratified(const TI *rowptr, int64_t nrows, int64_t ncols,
+                             TI *colind, uint64_t seed) {
+  const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x +
+                        threadIdx.x) / 64;
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / 64;
+  for (int64_t r = wave; r < nrows; r += nwaves) {
+    const int64_t b = rowptr[r], e = rowptr[r + 1], L = e - b;
+    for (int64_t i = lane; i < L; i += 64) {
+      const int64_t lo = (i * ncols) / L, hi = ((i + 1) * ncols) / L;
+      const uint64_t h = rnd(seed, static_cast<uint64_t>(b + i));
+      colind[b + i] = static_cast<TI>(lo + static_cast<int64_t>(
+                                               h % static_cast<uint64_t>(hi - lo)));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Laplacians (lexicographic rows, sorted columns) and band matrices.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void lap_coords(int dim, int64_t g, int64_t r,
+                                           int64_t c[3]) {
+  c[0] = c[1] = c[2] = 0;
+  if (dim == 2) {
+    c[0] = r / g;
+    c[1] = r % g;
+  } else {
+    c[0] = r / (g * g);
+    c[1] = (r / g) % g;
+    c[2] = r % g;
+  }
+}
+
+__global__ void k_lap_len(int dim, int64_t g, int64_t nrows, int64_t *rp64) {
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       r < nrows; r += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    int64_t c[3];
+    lap_coords(dim, g, r, c);
+    int64_t len = 1;
+    for (int d = 0; d < dim; d++) len += (c[d] > 0) + (c[d] < g - 1);
+    rp64[r + 1] = len;
+    if (r == 0) rp64[0] = 0;
+  }
+}
+
+template <typename TI, typename TV>
+__global__ void k_lap_fill(int dim, int64_t g, int64_t nrows,
+                           const TI *rowptr, TI *colind, TV *vals) {
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       r < nrows; r += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    int64_t c[3];
+    lap_coords(dim, g, r, c);
+    int64_t q = rowptr[r];
+    // strides of coordinates 0..dim-1 in row numbering
+    const int64_t st[3] = {dim == 2 ? g : g * g, dim == 2 ? 1 : g, 1};
+    // lower neighbours, from the farthest (largest stride) inward
+    for (int d = 0; d < dim; d++)
+      if (c[d] > 0) {
+        colind[q] = static_cast<TI>(r - st[d]);
+        vals[q++] = store_as<TV>(-1.0);
+      }
+    colind[q] = static_cast<TI>(r);
+    vals[q++] = store_as<TV>(2.0 * dim);
+    for (int d = dim - 1; d >= 0; d--)
+      if (c[d] < g - 1) {
+        colind[q] = static_cast<TI>(r + st[d]);
+        vals[q++] = store_as<TV>(-1.0);
+      }
+  }
+}
+
+__global__ void k_band_len(int64_t n, int64_t lo, int64_t hi, int64_t *rp64) {
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       r < n; r += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t a = r - lo < 0 ? 0 : r - lo;
+    const int64_t b = r + hi > n - 1 ? n - 1 : r + hi;
+    rp64[r + 1] = b - a + 1;
+    if (r == 0) rp64[0] = 0;
+  }
+}
+
+template <typename TI>
+__global__ void k_band_fill(int64_t n, int64_t lo, const TI *rowptr,
+                            TI *colind) {
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       r < n; r += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t a = r - lo < 0 ? 0 : r - lo;
+    const int64_t b = rowptr[r], e = rowptr[r + 1];
+    for (int64_t q = b; q < e; q++) colind[q] = static_cast<TI>(a + (q - b));
+  }
+}
+
+template <typename TS, typename TD>
+__global__ void k_convert(const TS *src, TD *dst, int64_t n) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       i < n; i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    double v;
+    if constexpr (std::is_same_v<TS, bf16> || std::is_same_v<TS, f16>)
+      v = static_cast<double>(to_f32(src[i]));
+    else
+      v = static_cast<double>(src[i]);
+    if constexpr (std::is_same_v<TD, bf16>)
+      dst[i] = f32_to_bf16(static_cast<float>(v));
+    else if constexpr (std::is_same_v<TD, f16>)
+      dst[i] = static_cast<f16>(static_cast<float>(v));
+    else
+      dst[i] = static_cast<TD>(v);
+  }
+}
+
+}  // namespace drx
+
+using namespace drx;
+
+extern "C" {
+
+int drx_tune_set(const char *key, int64_t value) {
+  return guarded([&] {
+    DRX_REQUIRE(key != nullptr, "key is null");
+    if (std::strcmp(key, "block_nnz") == 0) {
+      DRX_REQUIRE(value == 0 || (value >= 64 && value <= 8192 &&
+                                 (value & (value - 1)) == 0),
+                  "block_nnz must be 0 or a power of two in [64, 8192]");
+      g_block_nnz.store(value);
+    } else {
+      throw Error(DRX_ERR_INVALID, std::string("unknown tuning key: ") + key);
+    }
+  });
+}
+
+int drx_tune_get(const char *key, int64_t *value) {
+  return guarded([&] {
+    DRX_REQUIRE(key != nullptr && value != nullptr, "null argument");
+    if (std::strcmp(key, "block_nnz") == 0)
+      *value = default_block_nnz(DRX_I32);
+    else
+      throw Error(DRX_ERR_INVALID, std::string("unknown tuning key: ") + key);
+  });
+}
+
+int drx_gen_uniform(int dtype, void *out, int64_t n, double lo, double hi,
+                    uint64_t seed, drx_stream_t stream) {
+  return guarded([&] {
+    DRX_REQUIRE(n >= 0 && (n == 0 || out), "bad output");
+    if (n == 0) return;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    dim3 g = grid1d(n);
+    switch (dtype) {
+      case DRX_F64: hipLaunchKernelGGL(k_uniform<double>, g, dim3(256), 0, s, static_cast<double *>(out), n, lo, hi, seed); break;
+      case DRX_F32: hipLaunchKernelGGL(k_uniform<float>, g, dim3(256), 0, s, static_cast<float *>(out), n, lo, hi, seed); break;
+      case DRX_BF16: hipLaunchKernelGGL(k_uniform<bf16>, g, dim3(256), 0, s, static_cast<bf16 *>(out), n, lo, hi, seed); break;
+      case DRX_F16: hipLaunchKernelGGL(k_uniform<f16>, g, dim3(256), 0, s, static_cast<f16 *>(out), n, lo, hi, seed); break;
+      default: throw Error(DRX_ERR_INVALID, "bad dtype");
+    }
+    DRX_HIP(hipGetLastError());
+  });
+}
+
+int drx_gen_normal(int dtype, void *out, int64_t n, uint64_t seed,
+                   drx_stream_t stream) {
+  return guarded([&] {
+    DRX_REQUIRE(n >= 0 && (n == 0 || out), "bad output");
+    if (n == 0) return;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    dim3 g = grid1d(n);
+    switch (dtype) {
+      case DRX_F64: hipLaunchKernelGGL(k_normal<double>, g, dim3(256), 0, s, static_cast<double *>(out), n, seed); break;
+      case DRX_F32: hipLaunchKernelGGL(k_normal<float>, g, dim3(256), 0, s, static_cast<float *>(out), n, seed); break;
+      case DRX_BF16: hipLaunchKernelGGL(k_normal<bf16>, g, dim3(256), 0, s, static_cast<bf16 *>(out), n, seed); break;
+      case DRX_F16: hipLaunchKernelGGL(k_normal<f16>, g, dim3(256), 0, s, static_cast<f16 *>(out), n, seed); break;
+      default: throw Error(DRX_ERR_INVALID, "bad dtype");
+    }
+    DRX_HIP(hipGetLastError());
+  });
+}
+
+int drx_gen_zipf_rowptr(int itype, void *rowptr, int64_t nrows, double s_exp,
+                        int64_t lmax, uint64_t seed, int64_t *nnz_out,
+                        drx_stream_t stream) {
+  return guarded([&] {
+    DRX_REQUIRE(itype == DRX_I32 || itype == DRX_I64, "bad itype");
+    DRX_REQUIRE(nrows >= 1 && rowptr && nnz_out, "bad argument");
+    DRX_REQUIRE(s_exp > 1.0 && lmax >= 1 && lmax <= (1 << 24),
+                "need s > 1 and 1 <= lmax <= 2^24");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // cdf[l-1] = P(L <= l); cdf[lmax-1] = 1 (the clipped tail).
+    std::vector<double> cdf(static_cast<size_t>(lmax));
+    const double z = zeta(s_exp);
+    double acc = 0;
+    for (int64_t l = 1; l < lmax; l++) {
+      acc += std::pow(static_cast<double>(l), -s_exp) / z;
+      cdf[l - 1] = acc;
+    }
+    cdf[lmax - 1] = 2.0;  // every u in [0, 1) lands at or before lmax
+    double *d_cdf = nullptr;
+    int64_t *rp64 = nullptr;
+    DRX_HIP(hipMalloc(&d_cdf, lmax * sizeof(double)));
+    DRX_HIP(hipMalloc(&rp64, (nrows + 1) * sizeof(int64_t)));
+    try {
+      DRX_HIP(hipMemcpyAsync(d_cdf, cdf.data(), lmax * sizeof(double),
+                             hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_zipf_len, grid1d(nrows), dim3(256), 0, s, rp64,
+                         nrows, d_cdf, lmax, seed);
+      DRX_HIP(hipGetLastError());
+      *nnz_out = finish_rowptr(rp64, nrows, itype, rowptr, s);
+    } catch (...) {
+      (void)hipFree(d_cdf);
+      (void)hipFree(rp64);
+      throw;
+    }
+    (void)hipFree(d_cdf);
+    (void)hipFree(rp64);
+  });
+}
+
+int drx_gen_stratified_colind(int itype, const void *rowptr, int64_t nrows,
+                              int64_t ncols, void *colind, uint64_t seed,
+                              drx_stream_t stream) {
+  return guarded([&] {
+    DRX_REQUIRE(itype == DRX_I32 || itype == DRX_I64, "bad itype");
+    DRX_REQUIRE(nrows >= 0 && ncols >= 1 && ncols < (int64_t(1) << 40),
+                "bad size");
+    DRX_REQUIRE(nrows == 0 || (rowptr && colind), "null argument");
+    if (nrows == 0) return;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int64_t waves = std::min<int64_t>(nrows, int64_t(1) << 20);
+    dim3 g(static_cast<unsigned>((waves * 64 + 255) / 256));
+    if (itype == DRX_I32)
+      hipLaunchKernelGGL(k_stratified<int32_t>, g, dim3(256), 0, s,
+                         static_cast<const int32_t *>(rowptr), nrows, ncols,
+                         static_cast<int32_t *>(colind), seed);
+    else
+      hipLaunchKernelGGL(k_stratified<int64_t>, g, dim3(256), 0, s,
+                         static_cast<const int64_t *>(rowptr), nrows, ncols,
+                         static_cast<int64_t *>(colind), seed);
+    DRX_HIP(hipGetLastError());
+  });
+}
+
+int64_t drx_gen_laplacian_nnz(int dim, int64_t g) {
+  if (g < 1 || (dim != 2 && dim != 3)) return -1;
+  if (dim == 2) return 5 * g * g - 4 * g;
+  return 7 * g * g * g - 6 * g * g;
+}
+
+int drx_gen_laplacian(int dim, int64_t g, int itype, void *rowptr,
+                      void *colind, int vtype, void *vals,
+                      drx_stream_t stream) {
+  return guarded([&] {
+    DRX_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+    DRX_REQUIRE(g >= 1 && rowptr && colind && vals, "bad argument");
+    DRX_REQUIRE(itype == DRX_I32 || itype == DRX_I64, "bad itype");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int64_t nrows = dim == 2 ? g * g : g * g * g;
+    int64_t *rp64 = nullptr;
+    DRX_HIP(hipMalloc(&rp64, (nrows + 1) * sizeof(int64_t)));
+    try {
+      hipLaunchKernelGGL(k_lap_len, grid1d(nrows), dim3(256), 0, s, dim, g,
+                         nrows, rp64);
+      DRX_HIP(hipGetLastError());
+      const int64_t nnz = finish_rowptr(rp64, nrows, itype, rowptr, s);
+      DRX_REQUIRE(nnz == drx_gen_laplacian_nnz(dim, g), "laplacian nnz");
+#define DRX_LAP(TI, TV)                                                        \
+  hipLaunchKernelGGL((k_lap_fill<TI, TV>), grid1d(nrows), dim3(256), 0, s,     \
+                     dim, g, nrows, static_cast<const TI *>(rowptr),           \
+                     static_cast<TI *>(colind), static_cast<TV *>(vals))
+      const bool i32 = itype == DRX_I32;
+      switch (vtype) {
+        case DRX_F64: if (i32) DRX_LAP(int32_t, double); else DRX_LAP(int64_t, double); break;
+        case DRX_F32: if (i32) DRX_LAP(int32_t, float); else DRX_LAP(int64_t, float); break;
+        case DRX_BF16: if (i32) DRX_LAP(int32_t, bf16); else DRX_LAP(int64_t, bf16); break;
+        case DRX_F16: if (i32) DRX_LAP(int32_t, f16); else DRX_LAP(int64_t, f16); break;
+        default: throw Error(DRX_ERR_INVALID, "bad vtype");
+      }
+#undef DRX_LAP
+      DRX_HIP(hipGetLastError());
+      DRX_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+      (void)hipFree(rp64);
+      throw;
+    }
+    (void)hipFree(rp64);
+  });
+}
+
+int64_t drx_gen_band_nnz(int64_t n, int64_t lo, int64_t hi) {
+  if (n < 1 || lo < 0 || hi < 0) return -1;
+  int64_t total = 0;
+  // sum over rows of min(n-1, r+hi) - max(0, r-lo) + 1
+  const int64_t l = std::min(lo, n - 1), h = std::min(hi, n - 1);
+  total = n * (1 + l + h) - l * (l + 1) / 2 - h * (h + 1) / 2;
+  return total;
+}
+
+int drx_gen_band(int64_t n, int64_t lo, int64_t hi, int itype, void *rowptr,
+                 void *colind, int vtype, void *vals, uint64_t seed,
+                 drx_stream_t stream) {
+  return guarded([&] {
+    DRX_REQUIRE(n >= 1 && lo >= 0 && hi >= 0 && rowptr && colind,
+                "bad argument");
+    DRX_REQUIRE(itype == DRX_I32 || itype == DRX_I64, "bad itype");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int64_t *rp64 = nullptr;
+    DRX_HIP(hipMalloc(&rp64, (n + 1) * sizeof(int64_t)));
+    int64_t nnz = 0;
+    try {
+      hipLaunchKernelGGL(k_band_len, grid1d(n), dim3(256), 0, s, n, lo, hi,
+                         rp64);
+      DRX_HIP(hipGetLastError());
+      nnz = finish_rowptr(rp64, n, itype, rowptr, s);
+      DRX_REQUIRE(nnz == drx_gen_band_nnz(n, lo, hi), "band nnz");
+      if (itype == DRX_I32)
+        hipLaunchKernelGGL(k_band_fill<int32_t>, grid1d(n), dim3(256), 0, s, n,
+                           lo, static_cast<const int32_t *>(rowptr),
+                           static_cast<int32_t *>(colind));
+      else
+        hipLaunchKernelGGL(k_band_fill<int64_t>, grid1d(n), dim3(256), 0, s, n,
+                           lo, static_cast<const int64_t *>(rowptr),
+                           static_cast<int64_t *>(colind));
+      DRX_HIP(hipGetLastError());
+      DRX_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+      (void)hipFree(rp64);
+      throw;
+    }
+    (void)hipFree(rp64);
+    if (vals) {
+      int st = drx_gen_normal(vtype, vals, nnz, seed, stream);
+      if (st != DRX_OK) throw Error(st, drx_last_error());
+    }
+  });
+}
+
+int drx_convert(int src_dtype, const void *src, int dst_dtype, void *dst,
+                int64_t n, drx_stream_t stream) {
+  return guarded([&] {
+    DRX_REQUIRE(n >= 0 && (n == 0 || (src && dst)), "bad argument");
+    if (n == 0) return;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    dim3 g = grid1d(n);
+#define DRX_CV2(TS, TD)                                                        \
+  hipLaunchKernelGGL((k_convert<TS, TD>), g, dim3(256), 0, s,                  \
+                     static_cast<const TS *>(src), static_cast<TD *>(dst), n)
+#define DRX_CV1(TS)                                                            \
+  switch (dst_dtype) {                                                         \
+    case DRX_F64: DRX_CV2(TS, double); break;                                  \
+    case DRX_F32: DRX_CV2(TS, float); break;                                   \
+    case DRX_BF16: DRX_CV2(TS, bf16); break;                                   \
+    case DRX_F16: DRX_CV2(TS, f16); break;                                     \
+    default: throw Error(DRX_ERR_INVALID, "bad dst dtype");                    \
+  }
+    switch (src_dtype) {
+      case DRX_F64: DRX_CV1(double); break;
+      case DRX_F32: DRX_CV1(float); break;
+      case DRX_BF16: DRX_CV1(bf16); break;
+      case DRX_F16: DRX_CV1(f16); break;
+      default: throw Error(DRX_ERR_INVALID, "bad src dtype");
+    }
+#undef DRX_CV1
+#undef DRX_CV2
+    DRX_HIP(hipGetLastError());
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,246 @@
+"""Device-side CSR objects over the C ABI, with torch tensors as device
+memory (torch is plumbing here: allocation, streams, events).
+
+``CsrMatrix`` holds one CSR share in HBM (rowptr/colind/vals), its plan
+(row blocks for the kernel) and, on demand, its transposed structure for the
+adjoint. All math runs in libdrx_spmv.so; nothing here computes on the CPU.
+"""
+import ctypes
+
+import torch
+
+from ._lib import BF16, F16, F32, F64, I32, I64, DrxError, check, lib
+
+DTYPE_CODE = {torch.float64: F64, torch.float32: F32, torch.bfloat16: BF16,
+              torch.float16: F16}
+CODE_DTYPE = {v: k for k, v in DTYPE_CODE.items()}
+ITYPE_CODE = {torch.int32: I32, torch.int64: I64}
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _s(stream=None):
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def acc_dtype(vdtype):
+    return torch.float64 if vdtype == torch.float64 else torch.float32
+
+
+class Plan:
+    """Row-block analysis of one CSR share (drx_plan_create)."""
+
+    def __init__(self, rowptr, nrows, ncols, nnz, base=0, has_sentinel=True,
+                 block_nnz=0, stream=None):
+        if rowptr.device.type != "cuda":
+            raise ValueError("rowptr must be a device tensor")
+        self.rowptr = rowptr  # kept alive: the plan references it
+        self.itype = ITYPE_CODE[rowptr.dtype]
+        self.nrows, self.ncols, self.nnz = int(nrows), int(ncols), int(nnz)
+        self.base = int(base)
+        self.has_sentinel = bool(has_sentinel)
+        h = ctypes.c_void_p()
+        check(lib().drx_plan_create(ctypes.byref(h), self.itype, _p(rowptr),
+                                    self.base, self.nrows, self.ncols,
+                                    self.nnz, int(has_sentinel),
+                                    int(block_nnz), _s(stream)))
+        self.handle = h
+
+    def info(self):
+        nb, bn = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().drx_plan_info(self.handle, ctypes.byref(nb), ctypes.byref(bn)))
+        return nb.value, bn.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().drx_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Transpose:
+    """CSC of a share (drx_transpose_create) for the adjoint."""
+
+    def __init__(self, plan, colind, stream=None):
+        self.plan = plan
+        h = ctypes.c_void_p()
+        check(lib().drx_transpose_create(ctypes.byref(h), plan.handle,
+                                         _p(colind), _s(stream)))
+        self.handle = h
+
+    def arrays(self):
+        """(colptr, rowind, perm) copied to host numpy arrays (for tests)."""
+        cp, ri, pm = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib().drx_transpose_arrays(self.handle, ctypes.byref(cp),
+                                         ctypes.byref(ri), ctypes.byref(pm)))
+        idt = torch.int32 if self.plan.itype == I32 else torch.int64
+        out = []
+        for ptr, n in ((cp, self.plan.ncols + 1), (ri, self.plan.nnz),
+                       (pm, self.plan.nnz)):
+            t = torch.empty(n, dtype=idt, device="cuda")
+            if n:
+                torch.cuda.synchronize()
+                _copy_dev(t, ptr.value, n * t.element_size())
+            out.append(t)
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().drx_transpose_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _copy_dev(dst, src_ptr, nbytes):
+    """Device-to-device copy from a raw device pointer into tensor dst."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.restype = ctypes.c_int
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                              ctypes.c_int]
+    rc = hip.hipMemcpy(ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src_ptr),
+                       nbytes, 3)  # hipMemcpyDeviceToDevice
+    if rc != 0:
+        raise DrxError(-3, "hipMemcpy failed (%d)" % rc)
+
+
+class CsrMatrix:
+    """One CSR share resident in HBM.
+
+    rowptr: nrows (+1 with sentinel) absolute offsets; colind/vals: the
+    share's nnz entries; base: absolute offset of entry 0 of the share.
+    """
+
+    def __init__(self, rowptr, colind, vals, shape, base=0, has_sentinel=True,
+                 block_nnz=0, stream=None):
+        self.rowptr, self.colind, self.vals = rowptr, colind, vals
+        self.shape = (int(shape[0]), int(shape[1]))
+        self.nnz = int(colind.numel())
+        if colind.dtype != rowptr.dtype:
+            raise ValueError("rowptr and colind must share an index dtype")
+        if vals.numel() != self.nnz:
+            raise ValueError("vals and colind differ in length")
+        self.base = int(base)
+        self.has_sentinel = bool(has_sentinel)
+        self.plan = Plan(rowptr, self.shape[0], self.shape[1], self.nnz,
+                         base=base, has_sentinel=has_sentinel,
+                         block_nnz=block_nnz, stream=stream)
+        self._t = None
+
+    @property
+    def vcode(self):
+        return DTYPE_CODE[self.vals.dtype]
+
+    def matvec(self, x, y=None, alpha=1.0, beta=0.0, vals=None, stream=None):
+        """y = alpha*A*x + beta*y; x is (n,) or column-major (n, k) given as
+        a (k, n) contiguous tensor (row j = right-hand side j)."""
+        vals = self.vals if vals is None else vals
+        k, ldx = _cols(x, self.shape[1])
+        ydt = torch.float64 if vals.dtype == torch.float64 else torch.float32
+        if y is None:
+            shape = (self.shape[0],) if x.dim() == 1 else (k, self.shape[0])
+            y = torch.empty(shape, dtype=ydt, device=x.device)
+        _, ldy = _cols(y, self.shape[0])
+        check(lib().drx_spmv(self.plan.handle, DTYPE_CODE[vals.dtype],
+                             _p(self.colind), _p(vals), DTYPE_CODE[x.dtype],
+                             _p(x), ldx, DTYPE_CODE[y.dtype], _p(y), ldy, k,
+                             float(alpha), float(beta), _s(stream)))
+        return y
+
+    def transpose(self, stream=None):
+        if self._t is None:
+            self._t = Transpose(self.plan, self.colind, stream)
+        return self._t
+
+    def rmatvec(self, dy, dx=None, alpha=1.0, beta=0.0, vals=None, stream=None):
+        """dx = alpha*A^T*dy + beta*dx."""
+        vals = self.vals if vals is None else vals
+        t = self.transpose(stream)
+        k, lddy = _cols(dy, self.shape[0])
+        xdt = torch.float64 if vals.dtype == torch.float64 else torch.float32
+        if dx is None:
+            shape = (self.shape[1],) if dy.dim() == 1 else (k, self.shape[1])
+            dx = torch.empty(shape, dtype=xdt, device=dy.device)
+        _, lddx = _cols(dx, self.shape[1])
+        check(lib().drx_spmv_adjoint(t.handle, DTYPE_CODE[vals.dtype], _p(vals),
+                                     DTYPE_CODE[dy.dtype], _p(dy), lddy,
+                                     DTYPE_CODE[dx.dtype], _p(dx), lddx, k,
+                                     float(alpha), float(beta), _s(stream)))
+        return dx
+
+    def dvals(self, x, dy, out=None, alpha=1.0, beta=0.0, stream=None):
+        """out[e] = alpha * sum_j dy[row(e), j] x[col(e), j] + beta*out[e]."""
+        k, ldx = _cols(x, self.shape[1])
+        k2, lddy = _cols(dy, self.shape[0])
+        if k != k2:
+            raise ValueError("x and dy differ in k")
+        odt = torch.float64 if x.dtype == torch.float64 else torch.float32
+        if out is None:
+            out = torch.empty(self.nnz, dtype=odt, device=x.device)
+        check(lib().drx_csr_dvals(self.plan.handle, _p(self.colind),
+                                  DTYPE_CODE[x.dtype], _p(x), ldx,
+                                  DTYPE_CODE[dy.dtype], _p(dy), lddy, k,
+                                  DTYPE_CODE[out.dtype], _p(out), float(alpha),
+                                  float(beta), _s(stream)))
+        return out
+
+    def fused_residual(self, x, t, y=None, r=None, dvals=None, stream=None):
+        """y = A x, r = y - t, dvals[e] = r[row(e)] * x[col(e)] in one pass."""
+        ydt = torch.float64 if self.vals.dtype == torch.float64 else torch.float32
+        m = self.shape[0]
+        y = torch.empty(m, dtype=ydt, device=x.device) if y is None else y
+        r = torch.empty(m, dtype=ydt, device=x.device) if r is None else r
+        if dvals is None:
+            dvals = torch.empty(self.nnz, dtype=ydt, device=x.device)
+        check(lib().drx_spmv_fused_residual(
+            self.plan.handle, self.vcode, _p(self.colind), _p(self.vals),
+            DTYPE_CODE[x.dtype], _p(x), DTYPE_CODE[ydt], _p(t), _p(y), _p(r),
+            _p(dvals), _s(stream)))
+        return y, r, dvals
+
+    def close(self):
+        if self._t is not None:
+            self._t.close()
+            self._t = None
+        self.plan.close()
+
+
+def _cols(v, n):
+    """(k, leading dimension) of a (n,) or (k, n) operand."""
+    if v.dim() == 1:
+        if v.numel() < n:
+            raise ValueError("operand shorter than the matrix dimension")
+        return 1, max(n, 1)
+    if v.dim() == 2:
+        if v.shape[1] < n or v.stride(1) != 1:
+            raise ValueError("2-D operands are (k, n) with unit column stride")
+        return v.shape[0], v.stride(0)
+    raise ValueError("operand must be 1-D or 2-D")
+
+
+def from_host(rowptr, colind, vals, shape, device="cuda", itype=None,
+              vdtype=None, block_nnz=0):
+    """Upload host arrays (numpy or torch) as a full CSR matrix."""
+    rp = torch.as_tensor(rowptr)
+    ci = torch.as_tensor(colind)
+    va = torch.as_tensor(vals)
+    itype = itype or (torch.int64 if rp.dtype in (torch.int64, torch.uint64)
+                      else torch.int32)
+    rp = rp.to(torch.int64).to(itype).to(device)
+    ci = ci.to(torch.int64).to(itype).to(device)
+    va = va.to(vdtype or va.dtype).to(device)
+    return CsrMatrix(rp, ci, va, shape, base=0, has_sentinel=True,
+                     block_nnz=block_nnz)

@@ -1,0 +1,202 @@
+/*
+ * drx_spmv.h -- C ABI of the MI355X (gfx950) sparse gemv library.
+ *
+ * This is the drop-in boundary for Distributed Ranges' sparse gemv path: the
+ * per-share / per-tile kernel seam that dr::mp::gemv and dr::sp::gemv call.
+ * All pointers named rowptr/colind/vals/x/y/... are DEVICE pointers owned by
+ * the caller; the library owns only the opaque plan / transpose / batch
+ * objects. Every function returns an int status (DRX_OK == 0, negative on
+ * error), never throws and never aborts; drx_last_error() gives the message
+ * of the calling thread's last failure. Work is enqueued on `stream` (a
+ * hipStream_t; NULL = the null stream) and is asynchronous unless stated.
+ *
+ * Index semantics shared by every CSR entry point. A CSR share is
+ *   rowptr[0 .. nrows)        absolute offsets into the full matrix's nnz
+ *   rowptr[nrows]             read only when has_sentinel != 0
+ *   colind[0 .. nnz), vals[0 .. nnz)   the share's entries
+ *   rowptr_base               the absolute offset of colind[0] / vals[0]
+ * Row r covers share entries [clamp(rowptr[r] - base), clamp(end_r - base))
+ * with end_r = rowptr[r+1] (r + 1 < nrows or a sentinel) else base + nnz,
+ * and clamp() into [0, nnz]. This single rule is
+ *   - a full CSR matrix / dr::sp tile: base 0, sentinel present
+ *     (include/dr/views/csr_matrix_view.hpp:125-218,
+ *      include/dr/sp/containers/sparse_matrix.hpp:193-207);
+ *   - a dr::mp csr_row share: base = rowptr[first row], no sentinel
+ *     (include/dr/mp/containers/matrix_formats/csr_row_distribution.hpp:92-99);
+ *   - a dr::mp csr_eq share: rows cut at both ends of the nnz range, no
+ *     sentinel (include/dr/mp/containers/matrix_formats/csr_eq_distribution.hpp:
+ *     62-111), so the first and last outputs are partial sums.
+ * Dense operands with k right-hand sides are column-major like
+ * dr::mp::broadcasted_slim_matrix (include/dr/mp/containers/
+ * broadcasted_slim_matrix.hpp:123): element (i, j) of x is x[i + j*ldx].
+ */
+#ifndef DRX_SPMV_H
+#define DRX_SPMV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define DRX_OK 0
+#define DRX_ERR_INVALID (-1)     /* bad argument (null, size, dtype combo) */
+#define DRX_ERR_UNSUPPORTED (-2) /* valid but not built (dtype/index combo) */
+#define DRX_ERR_HIP (-3)         /* a HIP runtime call failed */
+#define DRX_ERR_NOMEM (-4)       /* device or host allocation failed */
+
+/* Element types (storage). Accumulation is f64 for DRX_F64 and f32 for the
+ * others; DRX_BF16 / DRX_F16 are storage-only types for vals and x. */
+#define DRX_F64 0
+#define DRX_F32 1
+#define DRX_BF16 2
+#define DRX_F16 3
+
+/* Index types of rowptr and colind (both the same type). */
+#define DRX_I32 0
+#define DRX_I64 1
+
+typedef void *drx_stream_t; /* a hipStream_t */
+typedef struct drx_plan_s *drx_plan;
+typedef struct drx_transpose_s *drx_transpose;
+
+/* ---- library ----------------------------------------------------------- */
+int drx_version(void);
+const char *drx_last_error(void);
+/* Process-wide tuning knobs. Key "block_nnz": default entries per row block
+ * of new plans (0 restores the built-in default; env DRX_BLOCK_NNZ seeds it).
+ * No counterpart in the reference. */
+int drx_tune_set(const char *key, int64_t value);
+int drx_tune_get(const char *key, int64_t *value);
+
+/* ---- plan: analysis of one CSR share ------------------------------------
+ * Splits the share's rows into blocks of about `block_nnz` entries (rows
+ * whose first entry falls in one block_nnz-sized window of the share; a row
+ * longer than the window is reduced by a whole workgroup). block_nnz <= 0
+ * picks the tuned default. Keeps a reference to `rowptr` (the caller keeps it
+ * alive and unchanged). Synchronizes `stream` once to validate the share's
+ * bounds.
+ * Replaces the per-share setup of csr_row_distribution::init
+ * (csr_row_distribution.hpp:205-318), csr_eq_distribution::init
+ * (csr_eq_distribution.hpp:220-333) and oneMKL's set_csr_data for the sp
+ * tile (include/dr/sp/algorithms/matrix/local_gemv.hpp:63-72). */
+int drx_plan_create(drx_plan *plan, int itype, const void *rowptr,
+                    int64_t rowptr_base, int64_t nrows, int64_t ncols,
+                    int64_t nnz, int has_sentinel, int64_t block_nnz,
+                    drx_stream_t stream);
+int drx_plan_destroy(drx_plan plan);
+/* Number of row blocks and the block size the plan uses. */
+int drx_plan_info(drx_plan plan, int64_t *nblocks, int64_t *block_nnz);
+
+/* ---- forward ------------------------------------------------------------
+ * y[:, j] = alpha * A x[:, j] + beta * y[:, j]   for j in [0, k)
+ * (beta == 0: y is not read). Deterministic (no atomics).
+ * Supported (vtype, xtype, ytype): (F64,F64,F64), (F32,F32,F32),
+ * (BF16,BF16,F32), (F16,F16,F32), (BF16,F32,F32), (F16,F32,F32).
+ * Replaces the kernels of csr_row_distribution::local_gemv
+ * (csr_row_distribution.hpp:63-134; beta = 0 into the zeroed partial),
+ * csr_eq_distribution::local_gemv (csr_eq_distribution.hpp:54-131) and
+ * dr::sp::__detail::local_gemv (local_gemv.hpp:83-98; alpha = beta = 1, as
+ * oneMKL sparse::gemv is called there). */
+int drx_spmv(drx_plan plan, int vtype, const void *colind, const void *vals,
+             int xtype, const void *x, int64_t ldx, int ytype, void *y,
+             int64_t ldy, int64_t k, double alpha, double beta,
+             drx_stream_t stream);
+
+/* One-shot form without a kept plan (plans on `stream`, runs, frees the plan
+ * stream-ordered; no host synchronization). Replaces
+ * dr::sp::__detail::mkl_gemv (local_gemv.hpp:59-78), which sets up the CSR
+ * handle on every call. */
+int drx_spmv_csr(int itype, const void *rowptr, int64_t rowptr_base,
+                 int64_t nrows, int64_t ncols, int64_t nnz, int has_sentinel,
+                 const void *colind, int vtype, const void *vals, int xtype,
+                 const void *x, int64_t ldx, int ytype, void *y, int64_t ldy,
+                 int64_t k, double alpha, double beta, drx_stream_t stream);
+
+/* ---- adjoint ------------------------------------------------------------
+ * No counterpart in the reference (it has no gradient path); these serve the
+ * differentiation of the gemv above.
+ * drx_transpose_create builds the transposed structure (CSC of the share) on
+ * the GPU by a stable sort of the entries by column: colptr[ncols+1],
+ * rowind[nnz] (row ids local to the share), perm[nnz] (the share position of
+ * each transposed entry) -- all in the plan's index type -- and a plan over
+ * it. vals are read through perm, so new vals need no rebuild. */
+int drx_transpose_create(drx_transpose *t, drx_plan plan, const void *colind,
+                         drx_stream_t stream);
+int drx_transpose_destroy(drx_transpose t);
+int drx_transpose_arrays(drx_transpose t, const void **colptr,
+                         const void **rowind, const void **perm);
+
+/* dx[:, j] = alpha * A^T dy[:, j] + beta * dx[:, j]. Same dtype combos as
+ * drx_spmv with (vtype, dytype, dxtype). Deterministic. */
+int drx_spmv_adjoint(drx_transpose t, int vtype, const void *vals, int dytype,
+                     const void *dy, int64_t lddy, int dxtype, void *dx,
+                     int64_t lddx, int64_t k, double alpha, double beta,
+                     drx_stream_t stream);
+
+/* dvals[e] = alpha * sum_j dy[row(e) + j*lddy] * x[col(e) + j*ldx]
+ *            + beta * dvals[e]   (the gradient of y = A x w.r.t. vals).
+ * dvtype is DRX_F64 (with F64 x, dy) or DRX_F32. */
+int drx_csr_dvals(drx_plan plan, const void *colind, int xtype, const void *x,
+                  int64_t ldx, int dytype, const void *dy, int64_t lddy,
+                  int64_t k, int dvtype, void *dvals, double alpha,
+                  double beta, drx_stream_t stream);
+
+/* Fused forward + residual + dvals for the loss 0.5 * ||A x - t||^2
+ * (k = 1): y = A x, r = y - t, dvals[e] = r[row(e)] * x[col(e)], in one
+ * pass over the matrix (vals/colind/x gathered once). r may alias y.
+ * Types as drx_spmv with ytype == t's type == r's type == dvals' type. */
+int drx_spmv_fused_residual(drx_plan plan, int vtype, const void *colind,
+                            const void *vals, int xtype, const void *x,
+                            int ytype, const void *t, void *y, void *r,
+                            void *dvals, drx_stream_t stream);
+
+/* ---- synthetic inputs (device-side generators) --------------------------
+ * Counter-based (splitmix64) so any box regenerates identical data.
+ * The reference generates its inputs on the host with
+ * dr::generate_random_csr / dr::generate_band_csr
+ * (include/dr/detail/generate_csr.hpp:40-153); these are device-side
+ * generators for the large inputs. */
+/* Fill n elements with U[lo, hi) (dtype F64/F32/BF16/F16). */
+int drx_gen_uniform(int dtype, void *out, int64_t n, double lo, double hi,
+                    uint64_t seed, drx_stream_t stream);
+/* Fill n elements with N(0, 1) (Box-Muller). */
+int drx_gen_normal(int dtype, void *out, int64_t n, uint64_t seed,
+                   drx_stream_t stream);
+/* Row lengths ~ Zipf(s) clipped to [1, lmax] into rowptr[1..nrows] and
+ * rowptr[0] = 0, then an inclusive scan: rowptr becomes a CSR row pointer.
+ * *nnz_out (host) receives rowptr[nrows]; fails with DRX_ERR_INVALID if it
+ * exceeds the index type. Synchronizes the stream. */
+int drx_gen_zipf_rowptr(int itype, void *rowptr, int64_t nrows, double s,
+                        int64_t lmax, uint64_t seed, int64_t *nnz_out,
+                        drx_stream_t stream);
+/* Given rowptr, fill colind with sorted, distinct columns in [0, ncols): the
+ * row's L entries take one uniform column in each of L equal strata of
+ * [0, ncols). Requires every row length <= ncols. */
+int drx_gen_stratified_colind(int itype, const void *rowptr, int64_t nrows,
+                              int64_t ncols, void *colind, uint64_t seed,
+                              drx_stream_t stream);
+/* 2-D (dim 2: grid g x g, 5-point) or 3-D (dim 3: g x g x g, 7-point)
+ * Laplacian: diagonal 2*dim, off-diagonals -1, rows in lexicographic order,
+ * columns sorted. nnz_of returns the entry count (host, no device work). */
+int64_t drx_gen_laplacian_nnz(int dim, int64_t g);
+int drx_gen_laplacian(int dim, int64_t g, int itype, void *rowptr,
+                      void *colind, int vtype, void *vals,
+                      drx_stream_t stream);
+/* Band matrix: row i has columns [max(0, i-lo), min(n-1, i+hi)], vals
+ * N(0,1) from `seed` (vals may be NULL to skip). */
+int64_t drx_gen_band_nnz(int64_t n, int64_t lo, int64_t hi);
+int drx_gen_band(int64_t n, int64_t lo, int64_t hi, int itype, void *rowptr,
+                 void *colind, int vtype, void *vals, uint64_t seed,
+                 drx_stream_t stream);
+/* Convert between storage dtypes (round to nearest even). */
+int drx_convert(int src_dtype, const void *src, int dst_dtype, void *dst,
+                int64_t n, drx_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DRX_SPMV_H */

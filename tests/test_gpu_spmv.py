@@ -1,0 +1,301 @@
+"""GPU parity of the forward kernel (drx_spmv / drx_spmv_csr) against the CPU
+oracle, through the C ABI.
+
+Tolerances (componentwise, scaled by |A||x|, against the extended-precision
+oracle): 1e-12 for FP64, 1e-5 for FP32; BF16/FP16 storage with FP32
+accumulation: 1e-5 against the oracle on the dequantized inputs (accumulation
+error only), total error vs the FP64 inputs reported and bounded by the
+storage rounding (2^-8 for BF16, 2^-11 for FP16, per unit of |A||x|).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+import drx
+from _oracle import componentwise_err, fp_equal, load_fixture
+
+pytestmark = pytest.mark.gpu
+
+TOL = {torch.float64: 1e-12, torch.float32: 1e-5}
+DEV = "cuda"
+
+
+def upload(rp, ci, v, shape, itype=torch.int32, vdtype=None, block_nnz=0):
+    return drx.from_host(np.asarray(rp, np.int64), np.asarray(ci, np.int64), v,
+                         shape, itype=itype, vdtype=vdtype, block_nnz=block_nnz)
+
+
+def np_rand_csr(m, n, density, seed, empty_rows=False):
+    rng = np.random.default_rng(seed)
+    A = sp.random(m, n, density=density, random_state=rng, format="csr")
+    A.data = rng.uniform(-1, 1, A.nnz)
+    if empty_rows and m > 3:
+        keep = (rng.random(m) > 0.3).astype(float)
+        A = (sp.diags(keep) @ A).tocsr()
+        A.eliminate_zeros()
+    A.sort_indices()
+    return A
+
+
+def run(A_dev, x_np, dtype, k_major=False):
+    x = torch.as_tensor(np.ascontiguousarray(x_np)).to(dtype).to(DEV)
+    y = A_dev.matvec(x)
+    torch.cuda.synchronize()
+    return y.double().cpu().numpy()
+
+
+@pytest.mark.parametrize("name", ["mp_random_100x100", "mp_random_1000x10",
+                                  "mp_random_10x1000"])
+@pytest.mark.parametrize("itype", [torch.int32, torch.int64])
+def test_reference_mp_fixtures(orc, name, itype):
+    """test/gtest/mp/gemv.cpp: x = 1 (fp_equal vs the reference's c_ref) and
+    the width-20 slim matrix b[i] = i."""
+    fx = load_fixture(name)
+    m, n = [int(t) for t in fx["shape"]]
+    A = upload(fx["rowptr"], fx["colind"], fx["vals"], (m, n), itype=itype)
+    y = run(A, np.ones(n, np.float32), torch.float32)
+    c_ref = orc.mp_gemv("eq", fx["rowptr"], fx["colind"], fx["vals"],
+                        np.ones(n, np.float32), (m, n), 1)
+    assert fp_equal(y.astype(np.float32), c_ref, 128 * np.finfo(np.float32).eps)
+    b = np.arange(n * 20, dtype=np.float32).reshape(20, n)
+    g = run(A, b, torch.float32)
+    ystar, absy = orc.spmv(fx["rowptr"], fx["colind"], fx["vals"], b)
+    assert componentwise_err(g, ystar, absy) < TOL[torch.float32]
+    gm = orc.mp_gemv("row", fx["rowptr"], fx["colind"], fx["vals"], b, (m, n), 2)
+    assert fp_equal(g.astype(np.float32), gm, 128 * np.finfo(np.float32).eps)
+
+
+@pytest.mark.parametrize("name,P", [("sp_gemv", 1), ("sp_gemv", 3),
+                                    ("sp_notsquare", 2), ("sp_notsquare_other", 4),
+                                    ("sp_verysparse", 1), ("sp_verysparse", 4)])
+def test_reference_sp_tiles_accumulate(orc, name, P):
+    """test/gtest/sp/gemv.cpp: each row tile i is generated with seed i; the
+    per-tile call accumulates (c += A_i b, alpha = beta = 1) into c = 0."""
+    m = {"sp_gemv": 100, "sp_notsquare": 10, "sp_notsquare_other": 1000,
+         "sp_verysparse": 100}[name]
+    n = {"sp_gemv": 100, "sp_notsquare": 1000, "sp_notsquare_other": 10,
+         "sp_verysparse": 100}[name]
+    tm = -(-m // P)
+    c = torch.zeros(m, dtype=torch.float32, device=DEV)
+    b = torch.ones(n, dtype=torch.float32, device=DEV)
+    c_ref = np.zeros(m, np.float32)
+    for i in range((m + tm - 1) // tm):
+        fx = load_fixture("%s_P%d_t%d" % (name, P, i))
+        rows = fx["rowptr"].size - 1
+        rp = torch.as_tensor(fx["rowptr"]).to(DEV)
+        ci = torch.as_tensor(fx["colind"]).to(DEV)
+        va = torch.as_tensor(fx["vals"]).to(DEV)
+        seg = c[i * tm:i * tm + rows]
+        drx.check(drx.lib().drx_spmv_csr(
+            drx.I64, drx.core._p(rp), 0, rows, n, ci.numel(), 1, drx.core._p(ci),
+            drx.F32, drx.core._p(va), drx.F32, drx.core._p(b), n, drx.F32,
+            drx.core._p(seg), rows, 1, 1.0, 1.0, drx.core._s()))
+        c_ref[i * tm:i * tm + rows] = orc.mp_gemv(
+            "eq", fx["rowptr"], fx["colind"], fx["vals"], np.ones(n, np.float32),
+            (rows, n), 1)
+    torch.cuda.synchronize()
+    assert fp_equal(c.cpu().numpy(), c_ref, 128 * np.finfo(np.float32).eps)
+
+
+def test_empty_and_zero_vector():
+    """sp EmptyGemv (no entries) and ZeroVector (dense a_ij = i + j, x = 0)."""
+    A = upload(np.zeros(101, np.int64), np.zeros(0, np.int64),
+               np.zeros(0, np.float32), (100, 100))
+    y = torch.full((100,), 7.0, device=DEV)
+    A.matvec(torch.ones(100, device=DEV), y=y)
+    torch.cuda.synchronize()
+    assert not y.any()
+    fx = load_fixture("sp_zerovector_dense")
+    A = upload(fx["rowptr"], fx["colind"], fx["vals"], (100, 100))
+    y = run(A, np.zeros(100, np.float32), torch.float32)
+    assert not y.any()
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("itype", [torch.int32, torch.int64])
+@pytest.mark.parametrize("block_nnz", [64, 256, 1024, 4096])
+@pytest.mark.parametrize("k", [1, 3, 16])
+def test_random_mixed_sign(orc, dtype, itype, block_nnz, k):
+    A = np_rand_csr(3000, 2500, 0.004, block_nnz + k, empty_rows=True)
+    Ad = upload(A.indptr, A.indices, A.data.astype(np.float64 if dtype == torch.float64
+                                                  else np.float32),
+                A.shape, itype=itype, block_nnz=block_nnz)
+    x = np.random.default_rng(k).uniform(-1, 1, (k, A.shape[1]))
+    xs = x.astype(np.float64 if dtype == torch.float64 else np.float32)
+    y = run(Ad, xs if k > 1 else xs[0], dtype)
+    ystar, absy = orc.spmv(A.indptr, A.indices, Ad.vals.cpu().numpy(), xs)
+    ystar = ystar if k > 1 else ystar[0]
+    absy = absy if k > 1 else absy[0]
+    assert componentwise_err(y, ystar, absy) < TOL[dtype]
+
+
+@pytest.mark.parametrize("lens", [[0, 0, 5000, 0, 3], [70000], [1, 9000, 1, 9000, 2],
+                                  [3000] * 7, [0] * 50])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_long_and_empty_rows(orc, lens, dtype):
+    """Rows longer than the LDS window go to the whole-workgroup path."""
+    rng = np.random.default_rng(len(lens))
+    n = 80000
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    ci = np.concatenate([np.sort(rng.choice(n, L, replace=False)) for L in lens]
+                        or [np.zeros(0)]).astype(np.int64)
+    v = rng.uniform(-1, 1, ci.size).astype(np.float64 if dtype == torch.float64 else np.float32)
+    for bn in (64, 1024):
+        Ad = upload(rp, ci, v, (len(lens), n), block_nnz=bn)
+        x = rng.uniform(-1, 1, n).astype(v.dtype)
+        y = run(Ad, x, dtype)
+        ystar, absy = orc.spmv(rp, ci, v, x)
+        assert componentwise_err(y, ystar, absy) < TOL[dtype]
+
+
+def test_alpha_beta_and_ldx(orc):
+    A = np_rand_csr(500, 400, 0.02, 1)
+    Ad = upload(A.indptr, A.indices, A.data, A.shape)
+    k, ldx, ldy = 3, 450, 520
+    x = torch.zeros(k, ldx, dtype=torch.float64, device=DEV)
+    x[:, :400] = torch.rand(k, 400, dtype=torch.float64, device=DEV) - 0.5
+    y0 = torch.rand(k, ldy, dtype=torch.float64, device=DEV)
+    y = y0.clone()
+    drx.check(drx.lib().drx_spmv(Ad.plan.handle, drx.F64, drx.core._p(Ad.colind),
+                                 drx.core._p(Ad.vals), drx.F64, drx.core._p(x), ldx,
+                                 drx.F64, drx.core._p(y), ldy, k, 2.0, -0.5,
+                                 drx.core._s()))
+    torch.cuda.synchronize()
+    ys, absy = orc.spmv(A.indptr, A.indices, A.data, x.cpu().numpy(), ldx=ldx)
+    expect = 2.0 * ys - 0.5 * y0[:, :500].cpu().numpy()
+    got = y[:, :500].cpu().numpy()
+    assert np.abs(got - expect).max() <= 1e-12 * (2 * absy + 0.5).max()
+    assert torch.equal(y[:, 500:], y0[:, 500:])  # untouched past nrows
+
+
+@pytest.mark.parametrize("kind", ["row", "eq"])
+@pytest.mark.parametrize("P", [1, 2, 3, 4, 7])
+def test_share_emulation(orc, kind, P):
+    """Each rank's share (absolute rowptr offsets, no sentinel) runs through
+    the kernel on the one GPU; the partials are assembled with the
+    reference's gather rules (csr_row_distribution.hpp:158-204,
+    csr_eq_distribution.hpp:158-216)."""
+    A = np_rand_csr(2000, 1500, 0.01, P, empty_rows=True)
+    rp = A.indptr.astype(np.int64)
+    ci = A.indices.astype(np.int64)
+    v = A.data.astype(np.float64)
+    m, n = A.shape
+    nnz = ci.size
+    k = 2
+    x = np.random.default_rng(P).uniform(-1, 1, (k, n))
+    xd = torch.as_tensor(x).to(DEV)
+    rp_d = torch.as_tensor(rp.astype(np.int32)).to(DEV)
+    ci_d = torch.as_tensor(ci.astype(np.int32)).to(DEV)
+    v_d = torch.as_tensor(v).to(DEV)
+    res = np.zeros((k, m))
+    if kind == "row":
+        seg, off, sz = orc.row_partition(rp, nnz, P)
+        for r in range(P):
+            if m <= seg * r:
+                continue
+            rows = min(seg, m - seg * r)
+            part = torch.zeros(k, seg, dtype=torch.float64, device=DEV)
+            sh = drx.CsrMatrix(rp_d[seg * r:seg * r + rows],
+                               ci_d[off[r]:off[r] + sz[r]], v_d[off[r]:off[r] + sz[r]],
+                               (rows, n), base=int(off[r]), has_sentinel=False)
+            sh.matvec(xd, y=part)
+            p = part.cpu().numpy()
+            res[:, seg * r:seg * r + rows] = p[:, :rows]
+    else:
+        seg, roff, rsz, _ = orc.eq_partition(rp, nnz, P)
+        res[:] = 0
+        for r in range(P):
+            if nnz <= seg * r:
+                continue
+            lo, hi = seg * r, min(seg * (r + 1), nnz)
+            part = torch.zeros(k, rsz[r], dtype=torch.float64, device=DEV)
+            sh = drx.CsrMatrix(rp_d[roff[r]:roff[r] + rsz[r]], ci_d[lo:hi], v_d[lo:hi],
+                               (int(rsz[r]), n), base=lo, has_sentinel=False)
+            sh.matvec(xd, y=part)
+            p = part.cpu().numpy()
+            res[:, roff[r]] += p[:, 0]
+            res[:, roff[r] + 1:roff[r] + rsz[r]] = p[:, 1:]
+    torch.cuda.synchronize()
+    expect = orc.mp_gemv(kind, rp, ci, v, x, (m, n), P)
+    _, absy = orc.spmv(rp, ci, v, x)
+    assert componentwise_err(res, expect, absy) < 1e-12
+
+
+def test_laplacian_2d_config1(orc):
+    """2-D 5-point Laplacian on 1000 x 1000 (1e6 rows, 4,996,000 entries),
+    FP64, x ~ U[-1, 1]: <= 1e-12 vs the oracle; x = 1 closed form."""
+    A = drx.gen.laplacian(2, 1000, vdtype=torch.float64)
+    assert A.nnz == 4996000
+    x = drx.gen.uniform(A.shape[1], torch.float64, -1, 1, seed=20261202)
+    y = A.matvec(x)
+    torch.cuda.synchronize()
+    rp, ci, v = A.rowptr.cpu().numpy(), A.colind.cpu().numpy(), A.vals.cpu().numpy()
+    ystar, absy = orc.spmv(rp, ci, v, x.cpu().numpy())
+    assert componentwise_err(y.cpu().numpy(), ystar, absy) < 1e-12
+    Y = A.matvec(torch.ones(A.shape[1], dtype=torch.float64, device=DEV)).cpu().numpy()
+    Y = Y.reshape(1000, 1000)
+    assert (Y[1:-1, 1:-1] == 0).all() and Y[0, 0] == 2 and (Y[0, 1:-1] == 1).all()
+
+
+def test_zipf_small_full_parity(orc):
+    A = drx.gen.zipf_csr(200000, 200000, s=1.5, lmax=4096, seed=5)
+    x = drx.gen.uniform(A.shape[1], torch.float32, -1, 1, seed=6)
+    y = A.matvec(x)
+    torch.cuda.synchronize()
+    rp, ci, v = A.rowptr.cpu().numpy(), A.colind.cpu().numpy(), A.vals.cpu().numpy()
+    ystar, absy = orc.spmv(rp, ci, v, x.cpu().numpy())
+    assert componentwise_err(y.cpu().numpy(), ystar, absy) < 1e-5
+
+
+@pytest.mark.parametrize("vdt", [torch.bfloat16, torch.float16])
+@pytest.mark.parametrize("xdt_same", [True, False])
+def test_low_precision_storage(orc, vdt, xdt_same):
+    A = drx.gen.laplacian(3, 40, vdtype=torch.float32)
+    x32 = drx.gen.uniform(A.shape[1], torch.float32, -1, 1, seed=3)
+    vq = A.vals.to(vdt)
+    xq = x32.to(vdt) if xdt_same else x32
+    y = A.matvec(xq, vals=vq)
+    torch.cuda.synchronize()
+    rp, ci = A.rowptr.cpu().numpy(), A.colind.cpu().numpy()
+    vd = vq.double().cpu().numpy()
+    xd = xq.double().cpu().numpy()
+    ystar, absy = orc.spmv(rp, ci, vd, xd)
+    assert componentwise_err(y.cpu().numpy(), ystar, absy) < 1e-5
+    y64, abs64 = orc.spmv(rp, ci, A.vals.double().cpu().numpy(), x32.double().cpu().numpy())
+    unit = 2.0 ** -8 if vdt == torch.bfloat16 else 2.0 ** -11
+    assert componentwise_err(y.cpu().numpy(), y64, abs64) < (2 if xdt_same else 1) * unit + 1e-5
+
+
+def test_zipf_config2_full_size_properties(orc):
+    """16M x 16M, Zipf(1.5) rows clipped to [1, 4096], FP32: 3050 sampled
+    rows (incl. the 50 longest) against the oracle, and exact
+    homogeneity y(2x) = 2 y(x) (scaling by 2 is exact in FP32)."""
+    n = 16_000_000
+    A = drx.gen.zipf_csr(n, n, s=1.5, lmax=4096, seed=0)
+    lens = torch.diff(A.rowptr.long())
+    assert int(lens.min()) >= 1 and int(lens.max()) <= 4096
+    mean = A.nnz / n
+    assert 90 < mean < 105, mean
+    x = drx.gen.uniform(n, torch.float32, -1, 1, seed=1)
+    y = A.matvec(x)
+    y2 = A.matvec(2 * x)
+    torch.cuda.synchronize()
+    assert torch.equal(y2, 2 * y)
+    rng = np.random.default_rng(0)
+    rows = np.concatenate([rng.choice(n, 3000, replace=False),
+                           torch.topk(lens, 50).indices.cpu().numpy()])
+    rp = A.rowptr.long().cpu().numpy()
+    # the sampled rows as a small CSR on the host, checked by the oracle
+    srp = [0]
+    sci, sv = [], []
+    for r in rows:
+        b, e = int(rp[r]), int(rp[r + 1])
+        sci.append(A.colind[b:e].cpu().numpy())
+        sv.append(A.vals[b:e].cpu().numpy())
+        srp.append(srp[-1] + e - b)
+    ystar, absy = orc.spmv(np.array(srp), np.concatenate(sci), np.concatenate(sv),
+                           x.cpu().numpy())
+    got = y.cpu().numpy()[rows]
+    assert componentwise_err(got, ystar, absy) < 1e-5
+    A.close()
