@@ -166,4 +166,40 @@ __device__ __forceinline__ int64_t row_begin(const TI *rowptr, int64_t base,
 // Number of XCDs that a launch's workgroups are dealt over round-robin.
 constexpr int kXcds = 8;
 
+// Workgroup i (dealt to XCD i % 8) takes logical block
+// (i % 8) * per_xcd + i / 8, so each XCD walks one contiguous range of rows.
+__device__ __forceinline__ int64_t map_block(int64_t i, int64_t per_xcd) {
+  return (i % kXcds) * per_xcd + i / kXcds;
+}
+
+// ---------------------------------------------------------------------------
+// Buffer addressing: a wave-uniform 128-bit resource (base, size) and 32-bit
+// per-lane byte offsets. Offsets at or past `bytes` read 0 (hardware range
+// check) instead of touching memory.
+// ---------------------------------------------------------------------------
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr int kRsrcWord3 = 0x00020000;  // raw buffer, gfx9 family
+constexpr int kNT = 2;                   // non-temporal (streamed once)
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void *base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0,
+                                           static_cast<int>(bytes), kRsrcWord3);
+}
+
+template <typename T, int AUX = 0>
+__device__ __forceinline__ T buf_load(rsrc_t r, uint32_t off) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, AUX));
+  else if constexpr (sizeof(T) == 4)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, AUX));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, AUX));
+}
+
+// Column index as a 32-bit element offset (the kernels that use buffer
+// gathers require ncols * sizeof(x) < 2^32).
+template <typename TI> __device__ __forceinline__ uint32_t col32(TI c) {
+  return static_cast<uint32_t>(c);
+}
+
 }  // namespace drx

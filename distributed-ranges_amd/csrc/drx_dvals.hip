@@ -1,0 +1,328 @@
+// drx_dvals.hip -- gradient of y = A x with respect to the stored values
+// (dvals), and the fused forward + residual + dvals pass, for gfx950.
+//
+// Both walk the plan's row blocks like the forward kernel. dvals uses lane
+// groups per row sized by the block's mean row length, so each group's
+// entries are contiguous (coalesced colind reads, dvals writes). The fused
+// kernel stages both the products and the gathered x values of the window in
+// LDS, so that after the row sums (and the residual r = y - t) the dvals of
+// the row are written from LDS without reading colind or x a second time.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "drx_common.hpp"
+#include "drx_internal.hpp"
+
+namespace drx {
+
+constexpr int kWG = 256;
+
+// ---------------------------------------------------------------------------
+// dvals[e] = alpha * sum_j dy[row(e) + j lddy] x[col(e) + j ldx] + beta dvals[e]
+// ---------------------------------------------------------------------------
+struct DvalsArgs {
+  const void *rowptr;
+  int64_t base, nrows, nnz;
+  int sent;
+  const BlockRec *rec;
+  int64_t nblocks, per_xcd, grid;
+  const void *colind, *x;
+  int64_t ldx;
+  const void *dy;
+  int64_t lddy, k;
+  void *dvals;
+  double alpha, beta;
+};
+
+template <int G, typename TX, typename TD, typename TA, typename TI>
+__device__ __forceinline__ void dvals_rows(const DvalsArgs &a, int64_t ra,
+                                           int64_t rb) {
+  const TI *rowptr = static_cast<const TI *>(a.rowptr);
+  const TI *colind = static_cast<const TI *>(a.colind);
+  const TX *x = static_cast<const TX *>(a.x);
+  const TX *dy = static_cast<const TX *>(a.dy);
+  TD *dv = static_cast<TD *>(a.dvals);
+  const TA alpha = static_cast<TA>(a.alpha), beta = static_cast<TA>(a.beta);
+  const int lane = threadIdx.x & (G - 1);
+  constexpr int NG = kWG / G;
+  for (int64_t r = ra + threadIdx.x / G; r < rb; r += NG) {
+    const int64_t s = row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r);
+    const int64_t e = row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r + 1);
+    for (int64_t q = s + lane; q < e; q += G) {
+      const int64_t col = static_cast<int64_t>(ld_stream(colind + q));
+      TA acc = 0;
+      for (int64_t j = 0; j < a.k; j++)
+        acc += to_acc<TA>(dy[r + j * a.lddy]) * to_acc<TA>(x[col + j * a.ldx]);
+      TA out = alpha * acc;
+      if (beta != TA(0)) out += beta * static_cast<TA>(dv[q]);
+      dv[q] = static_cast<TD>(out);
+    }
+  }
+}
+
+template <typename TX, typename TD, typename TA, typename TI>
+__global__ __launch_bounds__(kWG) void k_dvals(DvalsArgs a) {
+  for (int64_t wi = blockIdx.x; wi < a.per_xcd * kXcds; wi += a.grid) {
+    const int64_t blk = map_block(wi, a.per_xcd);
+    if (blk >= a.nblocks) continue;
+    const int64_t r0 = a.rec[blk].r0, r1 = a.rec[blk + 1].r0;
+    if (r0 >= r1) continue;
+    const int64_t avg = (a.rec[blk + 1].e0 - a.rec[blk].e0) / (r1 - r0);
+    if (avg <= 3)
+      dvals_rows<1, TX, TD, TA, TI>(a, r0, r1);
+    else if (avg <= 12)
+      dvals_rows<4, TX, TD, TA, TI>(a, r0, r1);
+    else if (avg <= 48)
+      dvals_rows<16, TX, TD, TA, TI>(a, r0, r1);
+    else
+      dvals_rows<64, TX, TD, TA, TI>(a, r0, r1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused forward + residual + dvals (k = 1)
+// ---------------------------------------------------------------------------
+struct FusedArgs {
+  const void *rowptr;
+  int64_t base, nrows, nnz;
+  int sent;
+  const BlockRec *rec;
+  int64_t nblocks, per_xcd, grid;
+  const void *colind, *vals, *x, *t;
+  void *y, *r, *dvals;
+};
+
+template <int G, typename TA, typename TY, typename TI>
+__device__ __forceinline__ void fused_rows(const TA *prod, const TA *xg,
+                                           const FusedArgs &a, int64_t ra,
+                                           int64_t rb, int64_t e0) {
+  const TI *rowptr = static_cast<const TI *>(a.rowptr);
+  const TY *t = static_cast<const TY *>(a.t);
+  TY *y = static_cast<TY *>(a.y);
+  TY *res = static_cast<TY *>(a.r);
+  TY *dv = static_cast<TY *>(a.dvals);
+  const int lane = threadIdx.x & (G - 1);
+  constexpr int NG = kWG / G;
+  for (int64_t r = ra + threadIdx.x / G; r < rb; r += NG) {
+    const int64_t s = row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r) - e0;
+    const int64_t e = row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r + 1) - e0;
+    TA acc = 0;
+    for (int64_t q = s + lane; q < e; q += G) acc += prod[q];
+    acc = group_sum<G>(acc);
+    const TA rr = acc - to_acc<TA>(t[r]);
+    if (lane == 0) {
+      y[r] = from_acc<TY>(acc);
+      res[r] = from_acc<TY>(rr);
+    }
+    for (int64_t q = s + lane; q < e; q += G) dv[e0 + q] = from_acc<TY>(rr * xg[q]);
+  }
+}
+
+template <int CAP, typename TV, typename TX, typename TY, typename TA,
+          typename TI>
+__global__ __launch_bounds__(kWG) void k_fused_residual(FusedArgs a) {
+  constexpr int T = CAP / kWG;
+  __shared__ TA prod[CAP];
+  __shared__ TA xg[CAP];
+  __shared__ TA red[kWG / 64];
+  const int tid = threadIdx.x;
+  const TI *colind = static_cast<const TI *>(a.colind);
+  const TV *vals = static_cast<const TV *>(a.vals);
+  const TX *x = static_cast<const TX *>(a.x);
+  for (int64_t wi = blockIdx.x; wi < a.per_xcd * kXcds; wi += a.grid) {
+    const int64_t blk = map_block(wi, a.per_xcd);
+    if (blk >= a.nblocks) continue;
+    const BlockRec b = a.rec[blk];
+    const int64_t r1 = a.rec[blk + 1].r0, e1 = a.rec[blk + 1].e0;
+    if (b.r0 >= r1) continue;
+    const int64_t last = r1 - 1;
+    const int64_t rs_end = b.es == e1 ? r1 : last;
+    const int n_s = static_cast<int>(b.es - b.e0);
+#pragma unroll
+    for (int u = 0; u < T; u++) {
+      const int idx = tid + u * kWG;
+      if (idx < n_s) {
+        const TA w = to_acc<TA>(ld_stream(vals + b.e0 + idx));
+        const TA xv = to_acc<TA>(x[static_cast<int64_t>(ld_stream(colind + b.e0 + idx))]);
+        xg[idx] = xv;
+        prod[idx] = w * xv;
+      }
+    }
+    __syncthreads();
+    const int64_t nr = rs_end - b.r0;
+    if (nr > 0) {
+      const int64_t avg = n_s / nr;
+      if (avg <= 3)
+        fused_rows<1, TA, TY, TI>(prod, xg, a, b.r0, rs_end, b.e0);
+      else if (avg <= 12)
+        fused_rows<4, TA, TY, TI>(prod, xg, a, b.r0, rs_end, b.e0);
+      else if (avg <= 48)
+        fused_rows<16, TA, TY, TI>(prod, xg, a, b.r0, rs_end, b.e0);
+      else
+        fused_rows<64, TA, TY, TI>(prod, xg, a, b.r0, rs_end, b.e0);
+    }
+    __syncthreads();
+    if (b.es < e1) {
+      TA acc = 0;
+      for (int64_t q = b.es + tid; q < e1; q += kWG)
+        acc += to_acc<TA>(ld_stream(vals + q)) *
+               to_acc<TA>(x[static_cast<int64_t>(ld_stream(colind + q))]);
+      acc = group_sum<64>(acc);
+      if ((tid & 63) == 0) red[tid >> 6] = acc;
+      __syncthreads();
+      TA s = 0;
+#pragma unroll
+      for (int i = 0; i < kWG / 64; i++) s += red[i];
+      const TA rr = s - to_acc<TA>(static_cast<const TY *>(a.t)[last]);
+      if (tid == 0) {
+        static_cast<TY *>(a.y)[last] = from_acc<TY>(s);
+        static_cast<TY *>(a.r)[last] = from_acc<TY>(rr);
+      }
+      TY *dv = static_cast<TY *>(a.dvals);
+      for (int64_t q = b.es + tid; q < e1; q += kWG)
+        dv[q] = from_acc<TY>(rr * to_acc<TA>(x[static_cast<int64_t>(colind[q])]));
+      __syncthreads();
+    }
+  }
+}
+
+void run_dvals(const Plan &p, const void *colind, int xtype, const void *x,
+               int64_t ldx, int dytype, const void *dy, int64_t lddy,
+               int64_t k, int dvtype, void *dvals, double alpha, double beta,
+               hipStream_t s) {
+  DRX_REQUIRE(k >= 1, "k must be >= 1");
+  DRX_REQUIRE(p.nnz == 0 || (colind && x && dy && dvals), "null operand");
+  DRX_REQUIRE(xtype == dytype, "x and dy must share a dtype");
+  if (p.nrows == 0 || p.nnz == 0) return;
+  DvalsArgs a{};
+  a.rowptr = p.rowptr;
+  a.base = p.base;
+  a.nrows = p.nrows;
+  a.nnz = p.nnz;
+  a.sent = p.has_sentinel;
+  a.rec = p.rec;
+  a.nblocks = p.nblocks;
+  const Grid g = grid_for(p.nblocks);
+  a.per_xcd = g.per_xcd;
+  a.grid = g.grid;
+  a.colind = colind;
+  a.x = x;
+  a.ldx = ldx;
+  a.dy = dy;
+  a.lddy = lddy;
+  a.k = k;
+  a.dvals = dvals;
+  a.alpha = alpha;
+  a.beta = beta;
+  const dim3 grid(static_cast<unsigned>(a.grid)), block(kWG);
+  const bool i32 = p.itype == DRX_I32;
+#define DRX_DV(TX, TD, TA)                                                     \
+  do {                                                                         \
+    if (i32)                                                                   \
+      hipLaunchKernelGGL((k_dvals<TX, TD, TA, int32_t>), grid, block, 0, s, a); \
+    else                                                                       \
+      hipLaunchKernelGGL((k_dvals<TX, TD, TA, int64_t>), grid, block, 0, s, a); \
+  } while (0)
+  if (xtype == DRX_F64 && dvtype == DRX_F64)
+    DRX_DV(double, double, double);
+  else if (xtype == DRX_F32 && dvtype == DRX_F32)
+    DRX_DV(float, float, float);
+  else if (xtype == DRX_BF16 && dvtype == DRX_F32)
+    DRX_DV(bf16, float, float);
+  else if (xtype == DRX_F16 && dvtype == DRX_F32)
+    DRX_DV(f16, float, float);
+  else
+    throw Error(DRX_ERR_UNSUPPORTED, "unsupported (xtype, dvtype)");
+#undef DRX_DV
+  DRX_HIP(hipGetLastError());
+}
+
+void run_fused(const Plan &p, int vtype, const void *colind, const void *vals,
+               int xtype, const void *x, int ytype, const void *t, void *y,
+               void *r, void *dvals, hipStream_t s) {
+  DRX_REQUIRE(p.nrows == 0 || (y && r && t), "null output");
+  DRX_REQUIRE(p.nnz == 0 || (colind && vals && x && dvals), "null operand");
+  DRX_REQUIRE(p.cap == 1024 || p.cap == 2048,
+              "the fused residual kernel needs a plan with block_nnz 512 or 1024");
+  if (p.nrows == 0) return;
+  FusedArgs a{};
+  a.rowptr = p.rowptr;
+  a.base = p.base;
+  a.nrows = p.nrows;
+  a.nnz = p.nnz;
+  a.sent = p.has_sentinel;
+  a.rec = p.rec;
+  a.nblocks = p.nblocks;
+  const Grid g = grid_for(p.nblocks);
+  a.per_xcd = g.per_xcd;
+  a.grid = g.grid;
+  a.colind = colind;
+  a.vals = vals;
+  a.x = x;
+  a.t = t;
+  a.y = y;
+  a.r = r;
+  a.dvals = dvals;
+  const dim3 grid(static_cast<unsigned>(a.grid)), block(kWG);
+  const bool i32 = p.itype == DRX_I32;
+  const bool small = p.cap == 1024;
+#define DRX_FU(TV, TX, TY, TA)                                                 \
+  do {                                                                         \
+    if (i32 && small)                                                          \
+      hipLaunchKernelGGL((k_fused_residual<1024, TV, TX, TY, TA, int32_t>),    \
+                         grid, block, 0, s, a);                                \
+    else if (i32)                                                              \
+      hipLaunchKernelGGL((k_fused_residual<2048, TV, TX, TY, TA, int32_t>),    \
+                         grid, block, 0, s, a);                                \
+    else if (small)                                                            \
+      hipLaunchKernelGGL((k_fused_residual<1024, TV, TX, TY, TA, int64_t>),    \
+                         grid, block, 0, s, a);                                \
+    else                                                                       \
+      hipLaunchKernelGGL((k_fused_residual<2048, TV, TX, TY, TA, int64_t>),    \
+                         grid, block, 0, s, a);                                \
+  } while (0)
+  if (vtype == DRX_F64 && xtype == DRX_F64 && ytype == DRX_F64)
+    DRX_FU(double, double, double, double);
+  else if (vtype == DRX_F32 && xtype == DRX_F32 && ytype == DRX_F32)
+    DRX_FU(float, float, float, float);
+  else if (vtype == DRX_BF16 && xtype == DRX_BF16 && ytype == DRX_F32)
+    DRX_FU(bf16, bf16, float, float);
+  else if (vtype == DRX_F16 && xtype == DRX_F16 && ytype == DRX_F32)
+    DRX_FU(f16, f16, float, float);
+  else
+    throw Error(DRX_ERR_UNSUPPORTED, "unsupported (vtype, xtype, ytype)");
+#undef DRX_FU
+  DRX_HIP(hipGetLastError());
+}
+
+}  // namespace drx
+
+using namespace drx;
+
+extern "C" {
+
+int drx_csr_dvals(drx_plan plan, const void *colind, int xtype, const void *x,
+                  int64_t ldx, int dytype, const void *dy, int64_t lddy,
+                  int64_t k, int dvtype, void *dvals, double alpha,
+                  double beta, drx_stream_t stream) {
+  return guarded([&] {
+    DRX_REQUIRE(plan != nullptr, "plan is null");
+    run_dvals(*reinterpret_cast<Plan *>(plan), colind, xtype, x, ldx, dytype,
+              dy, lddy, k, dvtype, dvals, alpha, beta,
+              static_cast<hipStream_t>(stream));
+  });
+}
+
+int drx_spmv_fused_residual(drx_plan plan, int vtype, const void *colind,
+                            const void *vals, int xtype, const void *x,
+                            int ytype, const void *t, void *y, void *r,
+                            void *dvals, drx_stream_t stream) {
+  return guarded([&] {
+    DRX_REQUIRE(plan != nullptr, "plan is null");
+    run_fused(*reinterpret_cast<Plan *>(plan), vtype, colind, vals, xtype, x,
+              ytype, t, y, r, dvals, static_cast<hipStream_t>(stream));
+  });
+}
+
+}  // extern "C"

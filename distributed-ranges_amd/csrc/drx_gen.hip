@@ -25,7 +25,7 @@ int64_t tuned_block_nnz() {
   if (v > 0) return v;
   if (const char *e = std::getenv("DRX_BLOCK_NNZ")) {
     long long w = std::atoll(e);
-    if (w >= 64 && w <= 8192 && (w & (w - 1)) == 0) return w;
+    if (w >= 64 && w <= 4096 && (w & (w - 1)) == 0) return w;
   }
   return 0;
 }
@@ -173,8 +173,9 @@ __global__ void k_stratified(const TI *rowptr, int64_t nrows, int64_t ncols,
     for (int64_t i = lane; i < L; i += 64) {
       const int64_t lo = (i * ncols) / L, hi = ((i + 1) * ncols) / L;
       const uint64_t h = rnd(seed, static_cast<uint64_t>(b + i));
+      const int64_t w = hi > lo ? hi - lo : 1;  // rows longer than ncols
       colind[b + i] = static_cast<TI>(lo + static_cast<int64_t>(
-                                               h % static_cast<uint64_t>(hi - lo)));
+                                               h % static_cast<uint64_t>(w)));
     }
   }
 }
@@ -282,9 +283,9 @@ int drx_tune_set(const char *key, int64_t value) {
   return guarded([&] {
     DRX_REQUIRE(key != nullptr, "key is null");
     if (std::strcmp(key, "block_nnz") == 0) {
-      DRX_REQUIRE(value == 0 || (value >= 64 && value <= 8192 &&
+      DRX_REQUIRE(value == 0 || (value >= 64 && value <= 4096 &&
                                  (value & (value - 1)) == 0),
-                  "block_nnz must be 0 or a power of two in [64, 8192]");
+                  "block_nnz must be 0 or a power of two in [64, 4096]");
       g_block_nnz.store(value);
     } else {
       throw Error(DRX_ERR_INVALID, std::string("unknown tuning key: ") + key);
@@ -492,6 +493,18 @@ int drx_gen_band(int64_t n, int64_t lo, int64_t hi, int itype, void *rowptr,
       int st = drx_gen_normal(vtype, vals, nnz, seed, stream);
       if (st != DRX_OK) throw Error(st, drx_last_error());
     }
+  });
+}
+
+int drx_copy_device(void *dst, const void *src, int64_t bytes,
+                    drx_stream_t stream) {
+  return guarded([&] {
+    DRX_REQUIRE(bytes >= 0 && (bytes == 0 || (dst && src)), "bad argument");
+    if (bytes == 0) return;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    DRX_HIP(hipMemcpyAsync(dst, src, static_cast<size_t>(bytes),
+                           hipMemcpyDefault, s));
+    DRX_HIP(hipStreamSynchronize(s));
   });
 }
 

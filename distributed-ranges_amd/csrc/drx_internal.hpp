@@ -7,19 +7,37 @@
 
 namespace drx {
 
-// Analysis of one CSR share: block_row[c] .. block_row[c+1] are the rows
-// whose first entry lies in [c*block_nnz, (c+1)*block_nnz) of the share.
+// One row block of a share: rows [r0, next.r0) whose first entry lies in
+// [c*block_nnz, (c+1)*block_nnz). Entries [e0, es) are reduced from the LDS
+// window; if es < next.e0 the block's last row is longer than the window and
+// entries [es, next.e0) are reduced by the whole workgroup. Record nblocks is
+// the end sentinel {nrows, end, end}.
+struct alignas(32) BlockRec {
+  int64_t r0, e0, es, pad;
+};
+
+// Analysis of one CSR share.
 struct Plan {
   int itype = 0;
   const void *rowptr = nullptr;
   int64_t base = 0, nrows = 0, ncols = 0, nnz = 0;
   int has_sentinel = 0;
   int64_t block_nnz = 1024;
+  int64_t cap = 2048;  // LDS window (entries) = 2 * block_nnz
   int64_t nblocks = 0;
-  int64_t *block_row = nullptr;  // device, nblocks + 1
+  BlockRec *rec = nullptr;  // device, nblocks + 1
   bool owns_async = false;
   int device = 0;
 };
+
+// Launch shape over a plan's blocks: grid workgroups walk per_xcd * 8
+// logical blocks (see map_block).
+struct Grid {
+  int64_t per_xcd, grid;
+};
+Grid grid_for(int64_t nblocks);
+
+inline size_t dtype_size(int t) { return t == 0 ? 8 : t == 1 ? 4 : 2; }
 
 int64_t tuned_block_nnz();
 int64_t default_block_nnz(int itype);

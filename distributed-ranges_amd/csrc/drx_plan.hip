@@ -1,0 +1,217 @@
+// drx_plan.hip -- the plan of one CSR share (row blocks and their entry
+// windows), library status strings, and the plan entry points.
+//
+// Block c holds the rows whose first entry lies in [c*NB, (c+1)*NB) of the
+// share (NB = block_nnz), found by one binary search on rowptr per block
+// boundary. Its record also stores where the LDS window ends (es): all of
+// the block's entries when they fit 2*NB, otherwise everything before the
+// block's last row, which is then longer than NB and is reduced by the whole
+// workgroup. Computing this once lets the kernels start streaming vals and
+// colind as soon as they read their 32-byte record.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "drx_common.hpp"
+#include "drx_internal.hpp"
+
+namespace drx {
+
+static thread_local std::string g_last_error;
+void set_last_error(const char *msg) { g_last_error = msg ? msg : ""; }
+const char *last_error() { return g_last_error.c_str(); }
+
+template <typename TI>
+__device__ int64_t first_row_at(const TI *rowptr, int64_t base, int64_t nrows,
+                                int64_t nnz, int sent, int64_t target) {
+  int64_t lo = 0, hi = nrows;  // first r with row_begin(r) >= target
+  while (lo < hi) {
+    int64_t mid = lo + (hi - lo) / 2;
+    if (row_begin(rowptr, base, nrows, nnz, sent, mid) >= target)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+
+template <typename TI>
+__global__ void k_plan_records(const TI *rowptr, int64_t base, int64_t nrows,
+                               int64_t nnz, int sent, int64_t nb_nnz,
+                               int64_t cap, int64_t nblocks, BlockRec *rec) {
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c > nblocks) return;
+  auto r_of = [&](int64_t cc) -> int64_t {
+    if (cc == 0) return 0;
+    if (cc >= nblocks) return nrows;
+    return first_row_at(rowptr, base, nrows, nnz, sent, cc * nb_nnz);
+  };
+  const int64_t r0 = r_of(c);
+  const int64_t e0 = row_begin(rowptr, base, nrows, nnz, sent, r0);
+  BlockRec out;
+  out.r0 = r0;
+  out.e0 = e0;
+  out.es = e0;
+  out.pad = 0;
+  if (c < nblocks) {
+    const int64_t r1 = r_of(c + 1);
+    const int64_t e1 = row_begin(rowptr, base, nrows, nnz, sent, r1);
+    if (r1 > r0 && e1 - e0 > cap)
+      out.es = row_begin(rowptr, base, nrows, nnz, sent, r1 - 1);
+    else
+      out.es = e1;
+  }
+  rec[c] = out;
+}
+
+int64_t default_block_nnz(int itype) {
+  int64_t v = tuned_block_nnz();
+  if (v > 0) return v;
+  (void)itype;
+  return 1024;
+}
+
+static void build_records(Plan &p, hipStream_t s, bool async_alloc) {
+  p.nblocks = p.nnz / p.block_nnz + 1;
+  p.cap = 2 * p.block_nnz;
+  const size_t bytes = static_cast<size_t>(p.nblocks + 1) * sizeof(BlockRec);
+  if (async_alloc)
+    DRX_HIP(hipMallocAsync(reinterpret_cast<void **>(&p.rec), bytes, s));
+  else
+    DRX_HIP(hipMalloc(reinterpret_cast<void **>(&p.rec), bytes));
+  p.owns_async = async_alloc;
+  const int tpb = 256;
+  const int64_t n = p.nblocks + 1;
+  const dim3 grid(static_cast<unsigned>((n + tpb - 1) / tpb));
+  if (p.itype == DRX_I32)
+    hipLaunchKernelGGL(k_plan_records<int32_t>, grid, dim3(tpb), 0, s,
+                       static_cast<const int32_t *>(p.rowptr), p.base, p.nrows,
+                       p.nnz, p.has_sentinel, p.block_nnz, p.cap, p.nblocks,
+                       p.rec);
+  else
+    hipLaunchKernelGGL(k_plan_records<int64_t>, grid, dim3(tpb), 0, s,
+                       static_cast<const int64_t *>(p.rowptr), p.base, p.nrows,
+                       p.nnz, p.has_sentinel, p.block_nnz, p.cap, p.nblocks,
+                       p.rec);
+  DRX_HIP(hipGetLastError());
+}
+
+static void validate_plan_args(int itype, const void *rowptr, int64_t nrows,
+                               int64_t ncols, int64_t nnz,
+                               int64_t block_nnz) {
+  DRX_REQUIRE(itype == DRX_I32 || itype == DRX_I64, "itype must be I32 or I64");
+  DRX_REQUIRE(nrows >= 0 && ncols >= 0 && nnz >= 0, "negative size");
+  DRX_REQUIRE(nrows == 0 || rowptr != nullptr, "rowptr is null");
+  DRX_REQUIRE(nrows > 0 || nnz == 0, "a share without rows has no entries");
+  DRX_REQUIRE(block_nnz <= 0 || (block_nnz >= 64 && block_nnz <= 4096 &&
+                                 (block_nnz & (block_nnz - 1)) == 0),
+              "block_nnz must be a power of two in [64, 4096]");
+  if (itype == DRX_I32) {
+    DRX_REQUIRE(nnz <= INT32_MAX && ncols <= static_cast<int64_t>(INT32_MAX) + 1,
+                "sizes exceed the 32-bit index type");
+  }
+}
+
+Plan *make_plan(int itype, const void *rowptr, int64_t base, int64_t nrows,
+                int64_t ncols, int64_t nnz, int sent, int64_t block_nnz,
+                hipStream_t s, bool validate, bool async_alloc) {
+  validate_plan_args(itype, rowptr, nrows, ncols, nnz, block_nnz);
+  Plan *p = new Plan();
+  p->itype = itype;
+  p->rowptr = rowptr;
+  p->base = base;
+  p->nrows = nrows;
+  p->ncols = ncols;
+  p->nnz = nnz;
+  p->has_sentinel = sent ? 1 : 0;
+  p->block_nnz = block_nnz > 0 ? block_nnz : default_block_nnz(itype);
+  try {
+    DRX_HIP(hipGetDevice(&p->device));
+    if (validate && nrows > 0) {
+      // The first and last row offsets must bracket the share.
+      const size_t isz = itype == DRX_I32 ? 4 : 8;
+      const int64_t lastidx = sent ? nrows : nrows - 1;
+      unsigned char buf[16] = {0};
+      DRX_HIP(hipMemcpyAsync(buf, rowptr, isz, hipMemcpyDeviceToHost, s));
+      DRX_HIP(hipMemcpyAsync(buf + 8,
+                             static_cast<const char *>(rowptr) + lastidx * isz,
+                             isz, hipMemcpyDeviceToHost, s));
+      DRX_HIP(hipStreamSynchronize(s));
+      int64_t first = 0, lastv = 0;
+      if (isz == 4) {
+        int32_t a, b;
+        std::memcpy(&a, buf, 4);
+        std::memcpy(&b, buf + 8, 4);
+        first = a;
+        lastv = b;
+      } else {
+        std::memcpy(&first, buf, 8);
+        std::memcpy(&lastv, buf + 8, 8);
+      }
+      DRX_REQUIRE(first - base <= nnz && lastv >= first,
+                  "rowptr does not bracket the share (check rowptr_base/nnz)");
+      if (sent)
+        DRX_REQUIRE(lastv - base <= nnz,
+                    "rowptr[nrows] - rowptr_base exceeds nnz");
+    }
+    build_records(*p, s, async_alloc);
+  } catch (...) {
+    delete p;
+    throw;
+  }
+  return p;
+}
+
+void free_plan(Plan *p, hipStream_t s) {
+  if (!p) return;
+  if (p->rec) {
+    if (p->owns_async)
+      (void)hipFreeAsync(p->rec, s);
+    else
+      (void)hipFree(p->rec);
+  }
+  delete p;
+}
+
+}  // namespace drx
+
+using namespace drx;
+
+extern "C" {
+
+int drx_version(void) { return 2; }
+
+const char *drx_last_error(void) { return drx::last_error(); }
+
+int drx_plan_create(drx_plan *out, int itype, const void *rowptr,
+                    int64_t rowptr_base, int64_t nrows, int64_t ncols,
+                    int64_t nnz, int has_sentinel, int64_t block_nnz,
+                    drx_stream_t stream) {
+  return guarded([&] {
+    DRX_REQUIRE(out != nullptr, "plan out-pointer is null");
+    *out = nullptr;
+    Plan *p = make_plan(itype, rowptr, rowptr_base, nrows, ncols, nnz,
+                        has_sentinel, block_nnz,
+                        static_cast<hipStream_t>(stream), true, false);
+    *out = reinterpret_cast<drx_plan>(p);
+  });
+}
+
+int drx_plan_destroy(drx_plan plan) {
+  return guarded([&] {
+    if (!plan) return;
+    free_plan(reinterpret_cast<Plan *>(plan), nullptr);
+  });
+}
+
+int drx_plan_info(drx_plan plan, int64_t *nblocks, int64_t *block_nnz) {
+  return guarded([&] {
+    DRX_REQUIRE(plan != nullptr, "plan is null");
+    Plan *p = reinterpret_cast<Plan *>(plan);
+    if (nblocks) *nblocks = p->nblocks;
+    if (block_nnz) *block_nnz = p->block_nnz;
+  });
+}
+
+}  // extern "C"

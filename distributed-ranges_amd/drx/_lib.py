@@ -63,6 +63,7 @@ SIGNATURES = {
     "drx_gen_band": (c_int, [c_i64, c_i64, c_i64, c_int, c_vp, c_vp, c_int,
                              c_vp, c_u64, c_vp]),
     "drx_convert": (c_int, [c_int, c_vp, c_int, c_vp, c_i64, c_vp]),
+    "drx_copy_device": (c_int, [c_vp, c_vp, c_i64, c_vp]),
 }
 
 

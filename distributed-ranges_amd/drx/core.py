@@ -106,15 +106,8 @@ class Transpose:
 
 
 def _copy_dev(dst, src_ptr, nbytes):
-    """Device-to-device copy from a raw device pointer into tensor dst."""
-    hip = ctypes.CDLL("libamdhip64.so")
-    hip.hipMemcpy.restype = ctypes.c_int
-    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
-                              ctypes.c_int]
-    rc = hip.hipMemcpy(ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src_ptr),
-                       nbytes, 3)  # hipMemcpyDeviceToDevice
-    if rc != 0:
-        raise DrxError(-3, "hipMemcpy failed (%d)" % rc)
+    """Copy from a raw device pointer into tensor dst (drx_copy_device)."""
+    check(lib().drx_copy_device(_p(dst), ctypes.c_void_p(src_ptr), nbytes, _s()))
 
 
 class CsrMatrix:

@@ -191,6 +191,11 @@ int64_t drx_gen_band_nnz(int64_t n, int64_t lo, int64_t hi);
 int drx_gen_band(int64_t n, int64_t lo, int64_t hi, int itype, void *rowptr,
                  void *colind, int vtype, void *vals, uint64_t seed,
                  drx_stream_t stream);
+/* Copy bytes between device (or host) buffers and wait for the copy; for
+ * bindings without a device runtime of their own (e.g. reading the arrays
+ * drx_transpose_arrays returns). */
+int drx_copy_device(void *dst, const void *src, int64_t bytes,
+                    drx_stream_t stream);
 /* Convert between storage dtypes (round to nearest even). */
 int drx_convert(int src_dtype, const void *src, int dst_dtype, void *dst,
                 int64_t n, drx_stream_t stream);

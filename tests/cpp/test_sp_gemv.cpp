@@ -1,0 +1,104 @@
+// test_sp_gemv.cpp -- the reference's dr::sp gemv tests
+// (test/gtest/sp/gemv.cpp:7-184) replayed through this host interface with
+// 1, 2, 3 and 4 logical devices on the one GPU (the reference's
+// --devicesCount duplication, test/gtest/sp/sp-tests.cpp:31-42).
+#include <vector>
+
+#include "dr/sp.hpp"
+#include "harness.hpp"
+
+namespace sp = dr::sp;
+
+template <typename T, typename I>
+void check_rowsums(long m, long k, sp::sparse_matrix<T, I> &a) {
+  sp::distributed_vector<T> b(k, 1.f);
+  sp::distributed_vector<T> c(m, 0.f);
+  sp::gemv(c, a, b);
+  std::vector<T> c_local(m);
+  sp::copy(c.begin(), c.end(), c_local.begin());
+  std::vector<T> c_ref(m, 0.f);
+  for (auto &&[index, v] : a) c_ref[index[0]] += v;
+  EXPECT_TRUE(harness::fp_equal(c_ref, c_local));
+}
+
+void random_case(long m, long k, double density) {
+  for (std::size_t P = 1; P <= 4; P++) {
+    sp::init(P);
+    sp::sparse_matrix<float> a({m, k}, density,
+                               sp::block_cyclic({sp::tile::div, sp::tile::div}, {sp::nprocs(), 1}));
+    check_rowsums(m, k, a);
+    sp::finalize();
+  }
+}
+
+TEST(SparseMatrix, Gemv) { random_case(100, 100, 0.1f); }
+TEST(SparseMatrix, NotSquareMatrix) { random_case(10, 1000, 0.1f); }
+TEST(SparseMatrix, NotSquareMatrixOtherAxis) { random_case(1000, 10, 0.1f); }
+TEST(SparseMatrix, VerySparseMatrix) { random_case(100, 100, 0.001f); }
+
+TEST(SparseMatrix, EmptyGemv) {
+  for (std::size_t P = 1; P <= 4; P++) {
+    sp::init(P);
+    std::vector<std::pair<std::pair<int, int>, float>> base;
+    auto csr = dr::__detail::convert_to_csr(base, {100, 100}, base.size(), std::allocator<float>{});
+    auto a = sp::create_distributed(csr, sp::row_cyclic());
+    sp::distributed_vector<float> b(100, 1.f);
+    sp::distributed_vector<float> c(100, 0.f);
+    sp::gemv(c, a, b);
+    std::vector<float> c_local(100);
+    sp::copy(c.begin(), c.end(), c_local.begin());
+    EXPECT_TRUE(harness::fp_equal(std::vector<float>(100, 0.f), c_local));
+    std::allocator<float>{}.deallocate(csr.values_data(), 0);
+    std::allocator<int>{}.deallocate(csr.rowptr_data(), 101);
+    std::allocator<int>{}.deallocate(csr.colind_data(), 0);
+    sp::finalize();
+  }
+}
+
+TEST(SparseMatrix, ZeroVector) {
+  for (std::size_t P = 1; P <= 4; P++) {
+    sp::init(P);
+    std::vector<std::pair<std::pair<int, int>, float>> base;
+    for (int i = 0; i < 100; i++)
+      for (int j = 0; j < 100; j++) base.push_back({{i, j}, static_cast<float>(i + j)});
+    auto csr = dr::__detail::convert_to_csr(base, {100, 100}, base.size(), std::allocator<float>{});
+    auto a = sp::create_distributed(csr, sp::row_cyclic());
+    sp::distributed_vector<float> b(100, 0.f);
+    sp::distributed_vector<float> c(100, 0.f);
+    sp::gemv(c, a, b);
+    std::vector<float> c_local(100);
+    sp::copy(c.begin(), c.end(), c_local.begin());
+    EXPECT_TRUE(harness::fp_equal(std::vector<float>(100, 0.f), c_local));
+    // and with b = 1: row sums of a_ij = i + j are 100 i + 4950 (exact)
+    sp::distributed_vector<float> b1(100, 1.f);
+    sp::distributed_vector<float> c1(100, 0.f);
+    sp::gemv(c1, a, b1);
+    sp::copy(c1.begin(), c1.end(), c_local.begin());
+    bool ok = true;
+    for (int i = 0; i < 100; i++) ok = ok && c_local[i] == 100.f * i + 4950.f;
+    EXPECT_TRUE(ok);
+    std::allocator<float>{}.deallocate(csr.values_data(), csr.size());
+    std::allocator<int>{}.deallocate(csr.rowptr_data(), 101);
+    std::allocator<int>{}.deallocate(csr.colind_data(), csr.size());
+    sp::finalize();
+  }
+}
+
+// gemv accumulates into c (alpha = beta = 1, as the reference's MKL call).
+TEST(SparseMatrix, GemvAccumulates) {
+  sp::init(2);
+  sp::sparse_matrix<float> a({100, 100}, 0.1f);
+  sp::distributed_vector<float> b(100, 1.f);
+  sp::distributed_vector<float> c(100, 0.f);
+  sp::gemv(c, a, b);
+  sp::gemv(c, a, b);
+  std::vector<float> c_local(100);
+  sp::copy(c.begin(), c.end(), c_local.begin());
+  std::vector<float> c_ref(100, 0.f);
+  for (auto &&[index, v] : a) c_ref[index[0]] += v;
+  for (auto &v : c_ref) v *= 2;
+  EXPECT_TRUE(harness::fp_equal(c_ref, c_local));
+  sp::finalize();
+}
+
+int main(int argc, char **argv) { return harness::run_all(argc > 1 ? argv[1] : nullptr); }

@@ -134,22 +134,24 @@ def test_autograd_finite_differences_fp64():
         r = A.matvec(x.detach()) - t
         assert torch.allclose(gx, A.rmatvec(r), rtol=0, atol=1e-12)
         assert torch.allclose(gv, A.dvals(x.detach(), r), rtol=0, atol=1e-12)
-    # central finite differences on sampled coordinates
+    # central finite differences on sampled coordinates. The loss is quadratic
+    # in every single coordinate, so the central difference has no truncation
+    # error; a large step keeps the cancellation error (eps*|L|/h) small.
     rng = np.random.default_rng(0)
-    h = 1e-6
+    h = 1e-3
     with torch.no_grad():
         for i in rng.choice(4096, 12, replace=False):
             xp, xm = x.detach().clone(), x.detach().clone()
             xp[i] += h
             xm[i] -= h
             fd = (loss_fn(vals.detach(), xp) - loss_fn(vals.detach(), xm)).item() / (2 * h)
-            assert abs(fd - gx[i].item()) <= 1e-6 * max(1.0, abs(fd))
+            assert abs(fd - gx[i].item()) <= 1e-8 * max(1.0, abs(fd))
         for e in rng.choice(A.nnz, 12, replace=False):
             vp, vm = vals.detach().clone(), vals.detach().clone()
             vp[e] += h
             vm[e] -= h
             fd = (loss_fn(vp, x.detach()) - loss_fn(vm, x.detach())).item() / (2 * h)
-            assert abs(fd - gv[e].item()) <= 1e-6 * max(1.0, abs(fd))
+            assert abs(fd - gv[e].item()) <= 1e-8 * max(1.0, abs(fd))
 
 
 def test_autograd_gradcheck_small():
