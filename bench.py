@@ -84,9 +84,21 @@ def headline(steps, warmup):
     log("zipf matrix: %d rows, %d nnz (mean %.2f/row), generated in %.1f s"
         % (n, A.nnz, A.nnz / n, time.perf_counter() - t0))
     B = spmv_bytes(n, n, A.nnz, 4, 4, 4, 4)
+    # the same CSR arrays through a plain plan (no panel layout), for reference
+    plain = drx.CsrMatrix(A.rowptr, A.colind, A.vals, A.shape)
+    _, plain_ms = time_launches(lambda: plain.matvec(x, y=y), max(3, steps // 4), 2)
+    plain.close()
+    t0 = time.perf_counter()
+    npan, width, nseg = A.set_panels(0)
+    log("panel layout: %d panels of %d columns, %d row segments, built in %.1f s"
+        % (npan, width, nseg, time.perf_counter() - t0))
     wall_ms, ev_ms = time_launches(lambda: A.matvec(x, y=y), steps, warmup)
-    log("zipf fp32: %.3f ms/step wall, %.3f ms/launch (events)" % (wall_ms, ev_ms))
-    return A, x, y, B, wall_ms, ev_ms
+    log("zipf fp32: %.3f ms/step wall, %.3f ms/step (events); plain CSR %.3f ms"
+        % (wall_ms, ev_ms, plain_ms))
+    layout = {"panels": npan, "panel_cols": width, "row_segments": nseg,
+              "plain_csr_ms_per_step": round(plain_ms, 4),
+              "plain_csr_GBps": round(B / plain_ms / 1e6, 2)}
+    return A, x, y, B, wall_ms, ev_ms, layout
 
 
 def cpu_baseline(A, x, y_gpu, target_s=10.0):
@@ -274,7 +286,7 @@ def main():
               % args.gpus, file=sys.stderr)
         sys.exit(2)
     torch.cuda.set_device(0)
-    A, x, y, B, wall_ms, ev_ms = headline(args.steps, args.warmup)
+    A, x, y, B, wall_ms, ev_ms, layout = headline(args.steps, args.warmup)
     value = B / (wall_ms * 1e-3) / 1e9
     achieved = B / (ev_ms * 1e-3) / 1e9
     result = {
@@ -287,11 +299,16 @@ def main():
                                "stratified-uniform columns, vals N(0,1), x U[-1,1]",
                    "rows": A.shape[0], "cols": A.shape[1], "nnz": A.nnz,
                    "mean_nnz_per_row": round(A.nnz / A.shape[0], 3),
-                   "algorithmic_bytes_per_step": B, "parallelism": "single GPU"},
+                   "algorithmic_bytes_per_step": B, "parallelism": "single GPU",
+                   "layout": "CSR + column-panel plan (2 MB x slices, L2-resident gathers)",
+                   **layout},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": read_traffic(), "kernel": "drx::k_spmv_fwd<2048,f32,i32>",
-                     "ms_per_launch": round(ev_ms, 4)},
+                     "traffic": read_traffic(),
+                     "kernel": "drx::k_spmv_fwd<256,2048,kRowMap,f32,i32> (one launch per "
+                               "column panel) + k_scale_y; achieved = bytes / summed time of "
+                               "one step's launches",
+                     "ms_per_step_kernels": round(ev_ms, 4)},
     }
     if not args.no_cpu:
         log("cpu baseline ...")

@@ -16,6 +16,8 @@ struct alignas(32) BlockRec {
   int64_t r0, e0, es, pad;
 };
 
+struct PanelSet;
+
 // Analysis of one CSR share.
 struct Plan {
   int itype = 0;
@@ -28,6 +30,7 @@ struct Plan {
   BlockRec *rec = nullptr;  // device, nblocks + 1
   bool owns_async = false;
   int device = 0;
+  PanelSet *panels = nullptr;  // column-panel layout (drx_plan_set_panels)
 };
 
 // Launch shape over a plan's blocks: grid workgroups walk per_xcd * 8
@@ -51,6 +54,16 @@ void run_spmv(const Plan &p, int vtype, const void *colind, const void *vals,
               int xtype, const void *x, int64_t ldx, int ytype, void *y,
               int64_t ldy, int64_t k, double alpha, double beta,
               hipStream_t s);
+
+void run_spmv_panels(const Plan &p, int vtype, const void *colind,
+                     const void *vals, int xtype, const void *x, int64_t ldx,
+                     int ytype, void *y, int64_t ldy, int64_t k, double alpha,
+                     double beta, hipStream_t s);
+void run_spmv_rowmap(const Plan &p, int vtype, const void *colind,
+                     const void *vals, const void *rowmap, int xtype,
+                     const void *x, int64_t ldx, int ytype, void *y,
+                     int64_t ldy, int64_t k, double alpha, hipStream_t s);
+void free_panels(PanelSet *ps);
 
 // Forward kernel reading vals through a permutation (vals[perm[e]]), used for
 // the transposed structure.

@@ -165,6 +165,7 @@ Plan *make_plan(int itype, const void *rowptr, int64_t base, int64_t nrows,
 
 void free_plan(Plan *p, hipStream_t s) {
   if (!p) return;
+  if (p->panels) free_panels(p->panels);
   if (p->rec) {
     if (p->owns_async)
       (void)hipFreeAsync(p->rec, s);

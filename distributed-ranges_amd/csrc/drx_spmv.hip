@@ -32,6 +32,11 @@
 
 namespace drx {
 
+// Kernel modes: plain CSR; vals read through a permutation (the transposed
+// structure of the adjoint); rows written through a row map and accumulated
+// (one column panel of a panel-layout plan).
+enum FwdMode { kPlain = 0, kPerm = 1, kRowMap = 2 };
+
 struct FwdArgs {
   const void *rowptr;
   int64_t base, nrows, nnz;
@@ -39,7 +44,8 @@ struct FwdArgs {
   const BlockRec *rec;
   int64_t nblocks, per_xcd, grid;
   const void *colind, *vals, *x;
-  const void *perm;  // PERM kernels read vals[perm[e]]
+  const void *perm;    // kPerm: vals[perm[e]]
+  const void *rowmap;  // kRowMap: output row of share row r
   int64_t ldx;
   uint32_t xbytes;   // bytes of one x column (buffer gathers)
   void *y;
@@ -63,12 +69,14 @@ __device__ __forceinline__ TA block_sum(TA v, TA *red) {
 // Rows [0, nr) of the block (row r0 + i), products prod[0 .. n_s), row
 // offsets in rp_s (relative to e0) or, for blocks with more rows than the
 // LDS holds, read from rowptr.
-template <int WG, int G, typename TA, typename TY, typename TI>
+template <int WG, int G, int MODE, typename TA, typename TY, typename TI>
 __device__ __forceinline__ void reduce_rows(const TA *prod, const int *rp_s,
+                                            const int64_t *map_s, const TA *ypre,
                                             bool rows_lds, const FwdArgs &a,
                                             int64_t r0, int64_t e0, int64_t nr,
                                             TY *yj, TA alpha, TA beta) {
   const TI *rowptr = static_cast<const TI *>(a.rowptr);
+  const TI *rowmap = static_cast<const TI *>(a.rowmap);
   const int lane = threadIdx.x & (G - 1);
   constexpr int NG = WG / G;
   for (int64_t i = threadIdx.x / G; i < nr; i += NG) {
@@ -84,22 +92,32 @@ __device__ __forceinline__ void reduce_rows(const TA *prod, const int *rp_s,
     for (int q = s + lane; q < e; q += G) acc += prod[q];
     acc = group_sum<G>(acc);
     if (lane == 0) {
-      TA out = alpha * acc;
-      if (beta != TA(0)) out += beta * to_acc<TA>(yj[r0 + i]);
-      yj[r0 + i] = from_acc<TY>(out);
+      if constexpr (MODE == kRowMap) {
+        if (ypre != nullptr && rows_lds) {
+          yj[map_s[i]] = from_acc<TY>(ypre[i] + alpha * acc);
+        } else {
+          const int64_t row = static_cast<int64_t>(rowmap[r0 + i]);
+          yj[row] = from_acc<TY>(to_acc<TA>(yj[row]) + alpha * acc);
+        }
+      } else {
+        TA out = alpha * acc;
+        if (beta != TA(0)) out += beta * to_acc<TA>(yj[r0 + i]);
+        yj[r0 + i] = from_acc<TY>(out);
+      }
     }
   }
 }
 
-template <int WG, typename TA, typename TY, typename TI>
+template <int WG, int MODE, typename TA, typename TY, typename TI>
 __device__ __forceinline__ void reduce_rows_any(int n_s, const TA *prod,
-                                                const int *rp_s, bool rows_lds,
+                                                const int *rp_s, const int64_t *map_s,
+                                                const TA *ypre, bool rows_lds,
                                                 const FwdArgs &a, int64_t r0,
                                                 int64_t e0, int64_t nr, TY *yj,
                                                 TA alpha, TA beta) {
   if (nr <= 0) return;
   const int64_t avg = n_s / nr;
-#define DRX_RR(G) reduce_rows<WG, G, TA, TY, TI>(prod, rp_s, rows_lds, a, r0, e0, nr, yj, alpha, beta)
+#define DRX_RR(G) reduce_rows<WG, G, MODE, TA, TY, TI>(prod, rp_s, map_s, ypre, rows_lds, a, r0, e0, nr, yj, alpha, beta)
   if (avg <= 3)
     DRX_RR(1);
   else if (avg <= 6)
@@ -117,22 +135,25 @@ __device__ __forceinline__ void reduce_rows_any(int n_s, const TA *prod,
 #undef DRX_RR
 }
 
-template <bool PERM, typename TV, typename TI>
+template <int MODE, typename TV, typename TI>
 __device__ __forceinline__ TV long_val(const TV *vals, const TI *perm,
                                        int64_t q) {
-  if constexpr (PERM)
+  if constexpr (MODE == kPerm)
     return vals[static_cast<int64_t>(ld_stream(perm + q))];
   else
     return ld_stream(vals + q);
 }
 
-template <int WG, int CAP, bool PERM, bool BUFX, typename TV, typename TX,
+template <int WG, int CAP, int MODE, bool BUFX, typename TV, typename TX,
           typename TY, typename TA, typename TI>
 __global__ __launch_bounds__(WG) void k_spmv_fwd(FwdArgs a) {
   constexpr int T = CAP / WG;
   constexpr int ROWCAP = CAP / 2;
+  constexpr int MAPCAP = MODE == kRowMap ? ROWCAP : 1;
   __shared__ TA prod[CAP];
   __shared__ int rp_s[ROWCAP + 1];
+  __shared__ int64_t map_s[MAPCAP];  // kRowMap: output rows of the block
+  __shared__ TA ypre[MAPCAP];        // kRowMap, k = 1: their current y
   __shared__ TA red[WG / 64];
   const int tid = threadIdx.x;
   const TI *rowptr = static_cast<const TI *>(a.rowptr);
@@ -158,7 +179,7 @@ __global__ __launch_bounds__(WG) void k_spmv_fwd(FwdArgs a) {
     uint32_t c[T];
     {
       const rsrc_t rc = make_rsrc(colind + b.e0, n_s * sizeof(TI));
-      if constexpr (PERM) {
+      if constexpr (MODE == kPerm) {
         const rsrc_t rp = make_rsrc(perm + b.e0, n_s * sizeof(TI));
 #pragma unroll
         for (int u = 0; u < T; u++) {
@@ -178,9 +199,17 @@ __global__ __launch_bounds__(WG) void k_spmv_fwd(FwdArgs a) {
       }
     }
     if (rows_lds) {
-      for (int i = tid; i <= nr; i += WG)
+      for (int i = tid; i <= nr; i += WG) {
         rp_s[i] = static_cast<int>(
             row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, b.r0 + i) - b.e0);
+        if constexpr (MODE == kRowMap) {
+          if (i < nr) {
+            const int64_t row = static_cast<int64_t>(static_cast<const TI *>(a.rowmap)[b.r0 + i]);
+            map_s[i] = row;
+            if (a.k == 1) ypre[i] = to_acc<TA>(y[row]);
+          }
+        }
+      }
     }
 
     for (int64_t j = 0; j < a.k; j++) {
@@ -200,7 +229,9 @@ __global__ __launch_bounds__(WG) void k_spmv_fwd(FwdArgs a) {
         }
       }
       __syncthreads();
-      reduce_rows_any<WG, TA, TY, TI>(n_s, prod, rp_s, rows_lds, a, b.r0, b.e0,
+      reduce_rows_any<WG, MODE, TA, TY, TI>(n_s, prod, rp_s, map_s,
+                                            MODE == kRowMap && a.k == 1 ? ypre : nullptr,
+                                            rows_lds, a, b.r0, b.e0,
                                       nr, y + j * a.ldy, alpha, beta);
       __syncthreads();
     }
@@ -213,10 +244,10 @@ __global__ __launch_bounds__(WG) void k_spmv_fwd(FwdArgs a) {
         TA acc0 = 0, acc1 = 0;
         int64_t q = b.es + tid;
         for (; q + 3 * WG < e1; q += 4 * WG) {
-          const TV w0 = long_val<PERM>(vals, perm, q);
-          const TV w1 = long_val<PERM>(vals, perm, q + WG);
-          const TV w2 = long_val<PERM>(vals, perm, q + 2 * WG);
-          const TV w3 = long_val<PERM>(vals, perm, q + 3 * WG);
+          const TV w0 = long_val<MODE>(vals, perm, q);
+          const TV w1 = long_val<MODE>(vals, perm, q + WG);
+          const TV w2 = long_val<MODE>(vals, perm, q + 2 * WG);
+          const TV w3 = long_val<MODE>(vals, perm, q + 3 * WG);
           const TI c0 = ld_stream(colind + q), c1 = ld_stream(colind + q + WG);
           const TI c2 = ld_stream(colind + q + 2 * WG), c3 = ld_stream(colind + q + 3 * WG);
           const TA x0 = to_acc<TA>(xj[static_cast<int64_t>(c0)]);
@@ -229,13 +260,18 @@ __global__ __launch_bounds__(WG) void k_spmv_fwd(FwdArgs a) {
           acc1 += to_acc<TA>(w3) * x3;
         }
         for (; q < e1; q += WG)
-          acc0 += to_acc<TA>(long_val<PERM>(vals, perm, q)) *
+          acc0 += to_acc<TA>(long_val<MODE>(vals, perm, q)) *
                   to_acc<TA>(xj[static_cast<int64_t>(ld_stream(colind + q))]);
         const TA s = block_sum<WG>(acc0 + acc1, red);
         if (tid == 0) {
-          TA out = alpha * s;
-          if (beta != TA(0)) out += beta * to_acc<TA>(y[last + j * a.ldy]);
-          y[last + j * a.ldy] = from_acc<TY>(out);
+          if constexpr (MODE == kRowMap) {
+            const int64_t row = static_cast<int64_t>(static_cast<const TI *>(a.rowmap)[last]);
+            y[row + j * a.ldy] = from_acc<TY>(to_acc<TA>(y[row + j * a.ldy]) + alpha * s);
+          } else {
+            TA out = alpha * s;
+            if (beta != TA(0)) out += beta * to_acc<TA>(y[last + j * a.ldy]);
+            y[last + j * a.ldy] = from_acc<TY>(out);
+          }
         }
       }
     }
@@ -254,50 +290,58 @@ Grid grid_for(int64_t nblocks) {
   return g;
 }
 
-template <bool PERM, bool BUFX, typename TV, typename TX, typename TY,
+template <int MODE, bool BUFX, typename TV, typename TX, typename TY,
           typename TA, typename TI>
 static void launch_fwd(const Plan &p, const FwdArgs &a, hipStream_t s) {
   const dim3 grid(static_cast<unsigned>(a.grid));
   if (p.cap <= 1024)
-    hipLaunchKernelGGL((k_spmv_fwd<256, 1024, PERM, BUFX, TV, TX, TY, TA, TI>),
+    hipLaunchKernelGGL((k_spmv_fwd<256, 1024, MODE, BUFX, TV, TX, TY, TA, TI>),
                        grid, dim3(256), 0, s, a);
   else if (p.cap <= 2048)
-    hipLaunchKernelGGL((k_spmv_fwd<256, 2048, PERM, BUFX, TV, TX, TY, TA, TI>),
+    hipLaunchKernelGGL((k_spmv_fwd<256, 2048, MODE, BUFX, TV, TX, TY, TA, TI>),
                        grid, dim3(256), 0, s, a);
   else if (p.cap <= 4096)
-    hipLaunchKernelGGL((k_spmv_fwd<512, 4096, PERM, BUFX, TV, TX, TY, TA, TI>),
+    hipLaunchKernelGGL((k_spmv_fwd<512, 4096, MODE, BUFX, TV, TX, TY, TA, TI>),
                        grid, dim3(512), 0, s, a);
   else
-    hipLaunchKernelGGL((k_spmv_fwd<1024, 8192, PERM, BUFX, TV, TX, TY, TA, TI>),
+    hipLaunchKernelGGL((k_spmv_fwd<1024, 8192, MODE, BUFX, TV, TX, TY, TA, TI>),
                        grid, dim3(1024), 0, s, a);
 }
 
-template <bool PERM, typename TV, typename TX, typename TY, typename TA>
+template <int MODE, typename TV, typename TX, typename TY, typename TA>
 static void launch_fwd_t(const Plan &p, const FwdArgs &a, bool bufx,
                          hipStream_t s) {
-  if (p.itype == DRX_I32) {
-    if (bufx)
-      launch_fwd<PERM, true, TV, TX, TY, TA, int32_t>(p, a, s);
+  if constexpr (MODE == kRowMap) {
+    // panels exist only when x fits a buffer resource
+    if (p.itype == DRX_I32)
+      launch_fwd<MODE, true, TV, TX, TY, TA, int32_t>(p, a, s);
     else
-      launch_fwd<PERM, false, TV, TX, TY, TA, int32_t>(p, a, s);
+      launch_fwd<MODE, true, TV, TX, TY, TA, int64_t>(p, a, s);
+    (void)bufx;
+  } else if (p.itype == DRX_I32) {
+    if (bufx)
+      launch_fwd<MODE, true, TV, TX, TY, TA, int32_t>(p, a, s);
+    else
+      launch_fwd<MODE, false, TV, TX, TY, TA, int32_t>(p, a, s);
   } else {
     if (bufx)
-      launch_fwd<PERM, true, TV, TX, TY, TA, int64_t>(p, a, s);
+      launch_fwd<MODE, true, TV, TX, TY, TA, int64_t>(p, a, s);
     else
-      launch_fwd<PERM, false, TV, TX, TY, TA, int64_t>(p, a, s);
+      launch_fwd<MODE, false, TV, TX, TY, TA, int64_t>(p, a, s);
   }
 }
 
-template <bool PERM>
+template <int MODE>
 static void run_spmv_impl(const Plan &p, int vtype, const void *colind,
-                          const void *vals, const void *perm, int xtype,
+                          const void *vals, const void *perm,
+                          const void *rowmap, int xtype,
                           const void *x, int64_t ldx, int ytype, void *y,
                           int64_t ldy, int64_t k, double alpha, double beta,
                           hipStream_t s) {
   DRX_REQUIRE(k >= 1, "k must be >= 1");
   DRX_REQUIRE(p.nrows == 0 || y != nullptr, "y is null");
   DRX_REQUIRE(p.nnz == 0 || (colind && vals && x), "colind/vals/x is null");
-  DRX_REQUIRE(!PERM || p.nnz == 0 || perm != nullptr, "perm is null");
+  DRX_REQUIRE(MODE != kPerm || p.nnz == 0 || perm != nullptr, "perm is null");
   DRX_REQUIRE(k == 1 || (ldx >= p.ncols && ldy >= p.nrows),
               "ldx/ldy smaller than the operand");
   if (p.nrows == 0) return;
@@ -315,6 +359,7 @@ static void run_spmv_impl(const Plan &p, int vtype, const void *colind,
   a.colind = colind;
   a.vals = vals;
   a.perm = perm;
+  a.rowmap = rowmap;
   a.x = x;
   a.ldx = ldx;
   a.y = y;
@@ -327,17 +372,17 @@ static void run_spmv_impl(const Plan &p, int vtype, const void *colind,
   const bool bufx = xb < (uint64_t(1) << 32);
   a.xbytes = bufx ? static_cast<uint32_t>(xb) : 0;
   if (vtype == DRX_F64 && xtype == DRX_F64 && ytype == DRX_F64)
-    launch_fwd_t<PERM, double, double, double, double>(p, a, bufx, s);
+    launch_fwd_t<MODE, double, double, double, double>(p, a, bufx, s);
   else if (vtype == DRX_F32 && xtype == DRX_F32 && ytype == DRX_F32)
-    launch_fwd_t<PERM, float, float, float, float>(p, a, bufx, s);
+    launch_fwd_t<MODE, float, float, float, float>(p, a, bufx, s);
   else if (vtype == DRX_BF16 && xtype == DRX_BF16 && ytype == DRX_F32)
-    launch_fwd_t<PERM, bf16, bf16, float, float>(p, a, bufx, s);
+    launch_fwd_t<MODE, bf16, bf16, float, float>(p, a, bufx, s);
   else if (vtype == DRX_F16 && xtype == DRX_F16 && ytype == DRX_F32)
-    launch_fwd_t<PERM, f16, f16, float, float>(p, a, bufx, s);
+    launch_fwd_t<MODE, f16, f16, float, float>(p, a, bufx, s);
   else if (vtype == DRX_BF16 && xtype == DRX_F32 && ytype == DRX_F32)
-    launch_fwd_t<PERM, bf16, float, float, float>(p, a, bufx, s);
+    launch_fwd_t<MODE, bf16, float, float, float>(p, a, bufx, s);
   else if (vtype == DRX_F16 && xtype == DRX_F32 && ytype == DRX_F32)
-    launch_fwd_t<PERM, f16, float, float, float>(p, a, bufx, s);
+    launch_fwd_t<MODE, f16, float, float, float>(p, a, bufx, s);
   else
     throw Error(DRX_ERR_UNSUPPORTED, "unsupported (vtype, xtype, ytype)");
   DRX_HIP(hipGetLastError());
@@ -347,8 +392,22 @@ void run_spmv(const Plan &p, int vtype, const void *colind, const void *vals,
               int xtype, const void *x, int64_t ldx, int ytype, void *y,
               int64_t ldy, int64_t k, double alpha, double beta,
               hipStream_t s) {
-  run_spmv_impl<false>(p, vtype, colind, vals, nullptr, xtype, x, ldx, ytype,
-                       y, ldy, k, alpha, beta, s);
+  if (p.panels) {
+    run_spmv_panels(p, vtype, colind, vals, xtype, x, ldx, ytype, y, ldy, k,
+                    alpha, beta, s);
+    return;
+  }
+  run_spmv_impl<kPlain>(p, vtype, colind, vals, nullptr, nullptr, xtype, x,
+                        ldx, ytype, y, ldy, k, alpha, beta, s);
+}
+
+// One column panel: rows through the panel's row map, accumulated into y.
+void run_spmv_rowmap(const Plan &p, int vtype, const void *colind,
+                     const void *vals, const void *rowmap, int xtype,
+                     const void *x, int64_t ldx, int ytype, void *y,
+                     int64_t ldy, int64_t k, double alpha, hipStream_t s) {
+  run_spmv_impl<kRowMap>(p, vtype, colind, vals, nullptr, rowmap, xtype, x,
+                         ldx, ytype, y, ldy, k, alpha, 1.0, s);
 }
 
 void run_spmv_perm(const Plan &p, int vtype, const void *colind,
@@ -356,8 +415,8 @@ void run_spmv_perm(const Plan &p, int vtype, const void *colind,
                    const void *x, int64_t ldx, int ytype, void *y,
                    int64_t ldy, int64_t k, double alpha, double beta,
                    hipStream_t s) {
-  run_spmv_impl<true>(p, vtype, colind, vals, perm, xtype, x, ldx, ytype, y,
-                      ldy, k, alpha, beta, s);
+  run_spmv_impl<kPerm>(p, vtype, colind, vals, perm, nullptr, xtype, x, ldx,
+                       ytype, y, ldy, k, alpha, beta, s);
 }
 
 }  // namespace drx

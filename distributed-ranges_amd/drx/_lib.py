@@ -35,6 +35,10 @@ SIGNATURES = {
                                 c_i64, c_i64, c_int, c_i64, c_vp]),
     "drx_plan_destroy": (c_int, [c_vp]),
     "drx_plan_info": (c_int, [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)]),
+    "drx_plan_set_panels": (c_int, [c_vp, c_int, c_vp, c_vp, c_i64, c_vp]),
+    "drx_plan_update_values": (c_int, [c_vp, c_vp, c_vp]),
+    "drx_plan_panels": (c_int, [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64),
+                                ctypes.POINTER(c_i64)]),
     "drx_spmv": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_vp, c_i64, c_int,
                          c_vp, c_i64, c_i64, c_dbl, c_dbl, c_vp]),
     "drx_spmv_csr": (c_int, [c_int, c_vp, c_i64, c_i64, c_i64, c_i64, c_int,

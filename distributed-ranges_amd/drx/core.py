@@ -153,6 +153,24 @@ class CsrMatrix:
                              float(alpha), float(beta), _s(stream)))
         return y
 
+    def set_panels(self, panel_cols=0, stream=None):
+        """Keep a column-panel copy of the entries in the plan (see
+        drx_plan_set_panels); later matvecs run one L2-sized panel at a time."""
+        check(lib().drx_plan_set_panels(self.plan.handle, self.vcode, _p(self.colind),
+                                        _p(self.vals), int(panel_cols), _s(stream)))
+        return self.panels()
+
+    def panels(self):
+        n, w, s = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        check(lib().drx_plan_panels(self.plan.handle, ctypes.byref(n), ctypes.byref(w),
+                                    ctypes.byref(s)))
+        return n.value, w.value, s.value
+
+    def update_values(self, vals, stream=None):
+        """New values for a plan with a panel layout (re-packed in place)."""
+        check(lib().drx_plan_update_values(self.plan.handle, _p(vals), _s(stream)))
+        self.vals = vals
+
     def transpose(self, stream=None):
         if self._t is None:
             self._t = Transpose(self.plan, self.colind, stream)

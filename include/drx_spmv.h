@@ -90,6 +90,24 @@ int drx_plan_destroy(drx_plan plan);
 /* Number of row blocks and the block size the plan uses. */
 int drx_plan_info(drx_plan plan, int64_t *nblocks, int64_t *block_nnz);
 
+/* ---- column-panel layout (optional) --------------------------------------
+ * For matrices whose x exceeds one XCD's L2 and whose columns are irregular
+ * (random 4-byte gathers beyond L2 run ~4x slower than within it): keeps a
+ * panel-major copy of (colind, vals) inside the plan -- columns cut into
+ * panels of panel_cols (0: ~2 MB of x per panel), rows compressed per panel
+ * -- and drx_spmv then runs one L2-resident launch per panel, accumulating
+ * into y in panel order (deterministic). drx_spmv on such a plan must be
+ * given the same colind/vals pointers; drx_plan_update_values re-packs new
+ * values through the kept permutation. Synchronizes the stream. No
+ * counterpart in the reference (an optimization of its local_gemv). */
+int drx_plan_set_panels(drx_plan plan, int vtype, const void *colind,
+                        const void *vals, int64_t panel_cols,
+                        drx_stream_t stream);
+int drx_plan_update_values(drx_plan plan, const void *vals,
+                           drx_stream_t stream);
+int drx_plan_panels(drx_plan plan, int64_t *npanels, int64_t *panel_cols,
+                    int64_t *segments);
+
 /* ---- forward ------------------------------------------------------------
  * y[:, j] = alpha * A x[:, j] + beta * y[:, j]   for j in [0, k)
  * (beta == 0: y is not read). Deterministic (no atomics).

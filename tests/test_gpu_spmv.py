@@ -299,3 +299,49 @@ def test_zipf_config2_full_size_properties(orc):
     got = y.cpu().numpy()[rows]
     assert componentwise_err(got, ystar, absy) < 1e-5
     A.close()
+
+
+@pytest.mark.parametrize("panel_cols", [64, 333, 1000])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("k", [1, 3])
+def test_panel_layout(orc, panel_cols, dtype, k):
+    """Column-panel layout: same products as plain CSR within tolerance, for
+    several panel widths (empty panels and rows included), k right-hand
+    sides, alpha/beta, and new values re-packed through the permutation."""
+    A = np_rand_csr(4000, 3000, 0.003, panel_cols + k, empty_rows=True)
+    v = A.data.astype(np.float64 if dtype == torch.float64 else np.float32)
+    Ad = upload(A.indptr, A.indices, v, A.shape, block_nnz=256)
+    npan, w, nseg = Ad.set_panels(panel_cols)
+    assert w == panel_cols and npan == -(-3000 // panel_cols) and nseg > 0
+    x = np.random.default_rng(k).uniform(-1, 1, (k, A.shape[1])).astype(v.dtype)
+    xd = torch.as_tensor(x if k > 1 else x[0]).to(DEV)
+    y0 = torch.rand((k, 4000) if k > 1 else (4000,), dtype=dtype, device=DEV)
+    y = y0.clone()
+    Ad.matvec(xd, y=y, alpha=2.0, beta=0.5)
+    torch.cuda.synchronize()
+    ys, absy = orc.spmv(A.indptr, A.indices, v, x if k > 1 else x[0])
+    expect = 2.0 * ys + 0.5 * y0.double().cpu().numpy()
+    tol = TOL[dtype]
+    den = 2.0 * absy + 0.5 * np.abs(y0.double().cpu().numpy())
+    assert (np.abs(y.double().cpu().numpy() - expect) <= tol * den + 1e-300).all()
+    # beta = 0 overwrites, new values through drx_plan_update_values
+    v2 = torch.as_tensor(v * -3).to(DEV)
+    Ad.update_values(v2)
+    y2 = Ad.matvec(xd)
+    torch.cuda.synchronize()
+    ys2, absy2 = orc.spmv(A.indptr, A.indices, v * -3, x if k > 1 else x[0])
+    assert componentwise_err(y2.double().cpu().numpy(), ys2, absy2) < tol
+    # the layout refuses other vals pointers (it holds a copy)
+    with pytest.raises(drx.DrxError):
+        Ad.matvec(xd, vals=torch.as_tensor(v).to(DEV))
+
+
+def test_panel_layout_zipf(orc):
+    A = drx.gen.zipf_csr(300000, 300000, s=1.5, lmax=4096, seed=9)
+    A.set_panels(32768)
+    x = drx.gen.uniform(A.shape[1], torch.float32, -1, 1, seed=10)
+    y = A.matvec(x)
+    torch.cuda.synchronize()
+    rp, ci, v = A.rowptr.cpu().numpy(), A.colind.cpu().numpy(), A.vals.cpu().numpy()
+    ystar, absy = orc.spmv(rp, ci, v, x.cpu().numpy())
+    assert componentwise_err(y.cpu().numpy(), ystar, absy) < 1e-5
