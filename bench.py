@@ -235,22 +235,27 @@ def form_lowprec(steps, zipf=None):
         out[name]["speedup_vs_f64"] = round(base_ms["f64"] / base_ms[name], 3)
     del A64
     if zipf is not None:
-        A, x = zipf
+        A, x = zipf  # FP32, with its panel layout
         vb = A.vals.to(torch.bfloat16)
         xb = x.to(torch.bfloat16)
+        Ab = drx.CsrMatrix(A.rowptr, A.colind, vb, A.shape)
+        Ab.set_panels(0)
         y = torch.empty(A.shape[0], dtype=torch.float32, device="cuda")
-        _, ms = time_launches(lambda: A.matvec(xb, y=y, vals=vb), steps, 3)
+        _, ms = time_launches(lambda: Ab.matvec(xb, y=y), steps, 3)
         _, ms32 = time_launches(lambda: A.matvec(x, y=y), steps, 3)
         B = spmv_bytes(A.shape[0], A.shape[1], A.nnz, 2, 4, 2, 4)
         y32 = A.matvec(x)
-        vd = A.vals.double()
-        A64z = drx.CsrMatrix(A.rowptr, A.colind, vd, A.shape)
+        yb = Ab.matvec(xb)
+        Ab.close()
+        del Ab, vb
+        A64z = drx.CsrMatrix(A.rowptr, A.colind, A.vals.double(), A.shape)
         yz64 = A64z.matvec(x.double())
-        del A64z, vd
+        A64z.close()
+        del A64z
         out["zipf_bf16"] = {"GBps": round(B / ms / 1e6, 1), "frac": round(B / ms / 1e6 / HBM_PEAK_GBS, 4),
-                            "us_per_launch": round(ms * 1e3, 1),
+                            "us_per_launch": round(ms * 1e3, 1), "layout": "column panels",
                             "speedup_vs_f32": round(ms32 / ms, 3),
-                            "rel_err_vs_fp64": rel_err(A.matvec(xb, vals=vb), yz64),
+                            "rel_err_vs_fp64": rel_err(yb, yz64),
                             "f32_rel_err_vs_fp64": rel_err(y32, yz64)}
     return out
 
