@@ -260,6 +260,60 @@ def form_lowprec(steps, zipf=None):
     return out
 
 
+def form_batched(steps, nmat=512, ks=(1, 4, 16)):
+    """512 independent square CSRs, rows U[1k, 200k], nnz/row U[3, 64], FP32,
+    k right-hand sides each: forward (y = A x) and adjoint (dx = A^T dy) of
+    every matrix in one persistent, autotuned, hipGraph-captured launch,
+    against the plain GPU form (one drx_spmv + one drx_spmv_adjoint call per
+    matrix)."""
+    rng = np.random.default_rng(0)
+    rows = rng.integers(1000, 200001, nmat)
+    mats = [drx.gen.uniform_rows_csr(int(m), int(m), 3, 64, seed=7000 + i)
+            for i, m in enumerate(rows)]
+    for A in mats:
+        A.transpose()
+    torch.cuda.synchronize()
+    nnz = sum(A.nnz for A in mats)
+    mtot = int(rows.sum())
+    out = {"matrices": nmat, "rows": mtot, "nnz": nnz, "dtype": "f32"}
+    for k in ks:
+        def vec(n, s):
+            return drx.gen.uniform(k * n, torch.float32, -1, 1, seed=s).reshape(k, n)
+        xs = [vec(A.shape[1], 1 + i) for i, A in enumerate(mats)]
+        dys = [vec(A.shape[0], 100000 + i) for i, A in enumerate(mats)]
+        ys = [torch.empty_like(x) for x in xs]
+        dxs = [torch.empty_like(d) for d in dys]
+        b = drx.Batch(forward=list(zip(mats, xs, ys)), adjoint=list(zip(mats, dys, dxs)), k=k)
+        tune = b.autotune(iters=2)
+        b.graph()
+        _, g_ms = time_launches(lambda: b.graph_launch(), steps, 2)
+
+        def plain():
+            for A, x, y, d, dx in zip(mats, xs, ys, dys, dxs):
+                A.matvec(x, y=y)
+                A.rmatvec(d, dx=dx)
+        _, p_ms = time_launches(plain, max(3, steps // 4), 1)
+        B = (spmv_bytes(mtot, mtot, nnz, 4, 4, 4, 4, k=k) +
+             spmv_bytes(mtot, mtot, nnz, 4, 4, 4, 4, k=k) + 4 * nnz)
+        # error of the first matrices against their FP64 forms
+        b.graph_launch()
+        errs = []
+        for A, x, y, d, dx in list(zip(mats, xs, ys, dys, dxs))[:8]:
+            A64 = drx.CsrMatrix(A.rowptr, A.colind, A.vals.double(), A.shape)
+            errs.append(rel_err(y, A64.matvec(x.double())))
+            errs.append(rel_err(dx, A64.rmatvec(d.double())))
+            A64.close()
+        out["k%d" % k] = {"GBps": round(B / g_ms / 1e6, 1), "frac": round(B / g_ms / 1e6 / HBM_PEAK_GBS, 4),
+                          "ms_fwd_plus_adjoint": round(g_ms, 3),
+                          "plain_gpu_form_ms": round(p_ms, 3),
+                          "speedup_vs_plain_gpu_form": round(p_ms / g_ms, 3),
+                          "tuning": tune, "max_rel_err_vs_fp64_first8": max(errs)}
+        b.close()
+        del xs, dys, ys, dxs
+        torch.cuda.empty_cache()
+    return out
+
+
 # ---------------------------------------------------------------------------
 def read_traffic():
     """HBM bytes per launch of the headline kernel from the committed
@@ -319,7 +373,7 @@ def main():
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(A, x, y, args.cpu_seconds)
     forms = {}
-    want = set(["laplacian2d_f64", "band_adjoint", "lowprec"]) if args.forms == "all" else \
+    want = set(["laplacian2d_f64", "band_adjoint", "lowprec", "batched"]) if args.forms == "all" else \
         set() if args.forms == "none" else set(args.forms.split(","))
     steps_f = max(5, min(args.steps, 20))
     if "lowprec" in want:
@@ -333,6 +387,9 @@ def main():
     if "band_adjoint" in want:
         log("form band_adjoint ...")
         forms["band_adjoint"] = form_band_adjoint(steps_f)
+    if "batched" in want:
+        log("form batched ...")
+        forms["batched"] = form_batched(steps_f)
     if forms:
         result["forms"] = forms
     print(json.dumps(result), flush=True)

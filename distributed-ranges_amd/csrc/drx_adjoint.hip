@@ -15,14 +15,6 @@
 
 namespace drx {
 
-struct Transpose {
-  Plan *plan = nullptr;     // over the CSC (rows of A^T = columns of A)
-  void *colptr = nullptr;   // ncols + 1
-  void *rowind = nullptr;   // nnz, row ids local to the share
-  void *perm = nullptr;     // nnz, share position of each CSC entry
-  int itype = 0;
-};
-
 template <typename TI>
 __global__ void k_row_ids(const TI *rowptr, int64_t base, int64_t nrows,
                           int64_t nnz, int sent, TI *rowid) {

@@ -146,6 +146,16 @@ __global__ void k_zipf_len(int64_t *rp64, int64_t nrows, const double *cdf,
   }
 }
 
+__global__ void k_uniform_len(int64_t *rp64, int64_t nrows, int64_t lmin,
+                              int64_t lmax, uint64_t seed) {
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       r < nrows; r += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t span = static_cast<uint64_t>(lmax - lmin + 1);
+    rp64[r + 1] = lmin + static_cast<int64_t>(rnd(seed, r) % span);
+    if (r == 0) rp64[0] = 0;
+  }
+}
+
 static double zeta(double s) {
   const int N = 100000;
   double acc = 0;
@@ -374,6 +384,29 @@ int drx_gen_zipf_rowptr(int itype, void *rowptr, int64_t nrows, double s_exp,
       throw;
     }
     (void)hipFree(d_cdf);
+    (void)hipFree(rp64);
+  });
+}
+
+int drx_gen_uniform_rowptr(int itype, void *rowptr, int64_t nrows,
+                           int64_t lmin, int64_t lmax, uint64_t seed,
+                           int64_t *nnz_out, drx_stream_t stream) {
+  return guarded([&] {
+    DRX_REQUIRE(itype == DRX_I32 || itype == DRX_I64, "bad itype");
+    DRX_REQUIRE(nrows >= 1 && rowptr && nnz_out, "bad argument");
+    DRX_REQUIRE(lmin >= 0 && lmax >= lmin, "need 0 <= lmin <= lmax");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int64_t *rp64 = nullptr;
+    DRX_HIP(hipMalloc(&rp64, (nrows + 1) * sizeof(int64_t)));
+    try {
+      hipLaunchKernelGGL(k_uniform_len, grid1d(nrows), dim3(256), 0, s, rp64,
+                         nrows, lmin, lmax, seed);
+      DRX_HIP(hipGetLastError());
+      *nnz_out = finish_rowptr(rp64, nrows, itype, rowptr, s);
+    } catch (...) {
+      (void)hipFree(rp64);
+      throw;
+    }
     (void)hipFree(rp64);
   });
 }

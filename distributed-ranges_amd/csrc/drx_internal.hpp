@@ -42,6 +42,15 @@ Grid grid_for(int64_t nblocks);
 
 inline size_t dtype_size(int t) { return t == 0 ? 8 : t == 1 ? 4 : 2; }
 
+// Transposed structure (CSC) of a share and a plan over it.
+struct Transpose {
+  Plan *plan = nullptr;    // over the CSC (rows of A^T = columns of A)
+  void *colptr = nullptr;  // ncols + 1
+  void *rowind = nullptr;  // nnz, row ids local to the share
+  void *perm = nullptr;    // nnz, share position of each CSC entry
+  int itype = 0;
+};
+
 int64_t tuned_block_nnz();
 int64_t default_block_nnz(int itype);
 

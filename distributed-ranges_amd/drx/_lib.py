@@ -71,6 +71,29 @@ SIGNATURES = {
 }
 
 
+class BatchItem(ctypes.Structure):
+    """drx_batch_item (include/drx_spmv.h)."""
+    _fields_ = [("plan", c_vp), ("transpose", c_vp), ("colind", c_vp),
+                ("vals", c_vp), ("x", c_vp), ("ldx", c_i64), ("y", c_vp),
+                ("ldy", c_i64)]
+
+
+SIGNATURES.update({
+    "drx_batch_create": (c_int, [ctypes.POINTER(c_vp), c_i64, ctypes.POINTER(BatchItem),
+                                 c_int, c_int, c_int, c_i64, c_dbl, c_dbl, c_vp]),
+    "drx_batch_destroy": (c_int, [c_vp]),
+    "drx_batch_run": (c_int, [c_vp, c_vp]),
+    "drx_batch_graph": (c_int, [c_vp]),
+    "drx_batch_graph_launch": (c_int, [c_vp, c_vp]),
+    "drx_batch_set_tuning": (c_int, [c_vp, c_i64, c_int, c_int, c_vp]),
+    "drx_batch_autotune": (c_int, [c_vp, c_int, ctypes.c_char_p, c_i64, c_vp]),
+    "drx_batch_info": (c_int, [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64),
+                               ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
+    "drx_gen_uniform_rowptr": (c_int, [c_int, c_vp, c_i64, c_i64, c_i64, c_u64,
+                                       ctypes.POINTER(c_i64), c_vp]),
+})
+
+
 class DrxError(RuntimeError):
     def __init__(self, status, message):
         super().__init__("drx status %d: %s" % (status, message))

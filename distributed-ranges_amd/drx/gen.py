@@ -63,3 +63,19 @@ def convert(src, dtype):
     check(lib().drx_convert(DTYPE_CODE[src.dtype], _p(src), DTYPE_CODE[dtype],
                             _p(out), src.numel(), _s()))
     return out
+
+
+def uniform_rows_csr(nrows, ncols, lmin, lmax, seed=0, vdtype=torch.float32,
+                     itype=torch.int32, block_nnz=0, device="cuda"):
+    """Row lengths uniform in [lmin, lmax]; stratified sorted distinct
+    columns; vals N(0,1)."""
+    rowptr = torch.empty(nrows + 1, dtype=itype, device=device)
+    nnz = ctypes.c_int64()
+    check(lib().drx_gen_uniform_rowptr(ITYPE_CODE[itype], _p(rowptr), nrows, lmin, lmax,
+                                       seed, ctypes.byref(nnz), _s()))
+    nnz = nnz.value
+    colind = torch.empty(nnz, dtype=itype, device=device)
+    check(lib().drx_gen_stratified_colind(ITYPE_CODE[itype], _p(rowptr), nrows, ncols,
+                                          _p(colind), seed + 1, _s()))
+    vals = normal(nnz, vdtype, seed=seed + 2, device=device)
+    return CsrMatrix(rowptr, colind, vals, (nrows, ncols), block_nnz=block_nnz)

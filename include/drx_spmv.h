@@ -61,6 +61,7 @@ extern "C" {
 typedef void *drx_stream_t; /* a hipStream_t */
 typedef struct drx_plan_s *drx_plan;
 typedef struct drx_transpose_s *drx_transpose;
+typedef struct drx_batch_s *drx_batch;
 
 /* ---- library ----------------------------------------------------------- */
 int drx_version(void);
@@ -171,6 +172,43 @@ int drx_spmv_fused_residual(drx_plan plan, int vtype, const void *colind,
                             int ytype, const void *t, void *y, void *r,
                             void *dvals, drx_stream_t stream);
 
+/* ---- batched products ---------------------------------------------------
+ * Many independent products -- forward items y = alpha*A x + beta*y and
+ * adjoint items dx = alpha*A^T dy + beta*dx -- in one persistent launch.
+ * Items re-plan their matrices inside the batch with the batch's tuning
+ * (block_nnz, gshift, wg_per_cu); all items share k, alpha, beta and the
+ * dtype combination (as drx_spmv) and an index type. A run allocates and
+ * clears nothing, so drx_batch_graph can capture it into a hipGraph
+ * (instantiated once; drx_batch_graph_launch replays it on any stream).
+ * drx_batch_autotune times every tuning combination on the batch and keeps
+ * the fastest (invalidates a captured graph); its JSON describes the choice.
+ * No counterpart in the reference: it batches the per-share gemv seam. */
+typedef struct drx_batch_item {
+  drx_plan plan;           /* forward item: the matrix's plan (else NULL) */
+  drx_transpose transpose; /* adjoint item: its transposed structure (else NULL) */
+  const void *colind;      /* forward: the matrix's colind (adjoint: unused) */
+  const void *vals;        /* the matrix's values, CSR order */
+  const void *x;           /* forward: x; adjoint: dy */
+  int64_t ldx;             /* <= 0: the operand's length */
+  void *y;                 /* forward: y; adjoint: dx */
+  int64_t ldy;             /* <= 0: the result's length */
+} drx_batch_item;
+
+int drx_batch_create(drx_batch *batch, int64_t nitems,
+                     const drx_batch_item *items, int vtype, int xtype,
+                     int ytype, int64_t k, double alpha, double beta,
+                     drx_stream_t stream);
+int drx_batch_destroy(drx_batch batch);
+int drx_batch_run(drx_batch batch, drx_stream_t stream);
+int drx_batch_graph(drx_batch batch);
+int drx_batch_graph_launch(drx_batch batch, drx_stream_t stream);
+int drx_batch_set_tuning(drx_batch batch, int64_t block_nnz, int gshift,
+                         int wg_per_cu, drx_stream_t stream);
+int drx_batch_autotune(drx_batch batch, int iters, char *json,
+                       int64_t json_cap, drx_stream_t stream);
+int drx_batch_info(drx_batch batch, int64_t *work_units, int64_t *block_nnz,
+                   int *gshift, int *wg_per_cu);
+
 /* ---- synthetic inputs (device-side generators) --------------------------
  * Counter-based (splitmix64) so any box regenerates identical data.
  * The reference generates its inputs on the host with
@@ -190,6 +228,11 @@ int drx_gen_normal(int dtype, void *out, int64_t n, uint64_t seed,
 int drx_gen_zipf_rowptr(int itype, void *rowptr, int64_t nrows, double s,
                         int64_t lmax, uint64_t seed, int64_t *nnz_out,
                         drx_stream_t stream);
+/* Row lengths uniform in [lmin, lmax] (inclusive), scanned into rowptr as
+ * drx_gen_zipf_rowptr does. */
+int drx_gen_uniform_rowptr(int itype, void *rowptr, int64_t nrows,
+                           int64_t lmin, int64_t lmax, uint64_t seed,
+                           int64_t *nnz_out, drx_stream_t stream);
 /* Given rowptr, fill colind with sorted, distinct columns in [0, ncols): the
  * row's L entries take one uniform column in each of L equal strata of
  * [0, ncols). Requires every row length <= ncols. */

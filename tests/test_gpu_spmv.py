@@ -345,3 +345,74 @@ def test_panel_layout_zipf(orc):
     rp, ci, v = A.rowptr.cpu().numpy(), A.colind.cpu().numpy(), A.vals.cpu().numpy()
     ystar, absy = orc.spmv(rp, ci, v, x.cpu().numpy())
     assert componentwise_err(y.cpu().numpy(), ystar, absy) < 1e-5
+
+
+def _batch_mats(nmat, seed, rows=(500, 4000), lens=(3, 64)):
+    rng = np.random.default_rng(seed)
+    mats = []
+    for i in range(nmat):
+        m = int(rng.integers(*rows))
+        mats.append(drx.gen.uniform_rows_csr(m, m, lens[0], lens[1], seed=seed * 1000 + i,
+                                             block_nnz=1024))
+    return mats
+
+
+@pytest.mark.parametrize("k", [1, 4])
+def test_batch_matches_single_calls_and_oracle(orc, k):
+    """Forward + adjoint items of uneven matrices in one persistent launch:
+    bit-identical to the single calls at the same block size and lane-group
+    thresholds, and within tolerance of the oracle."""
+    mats = _batch_mats(9, 3 + k)
+    def vec(n, s):
+        return drx.gen.uniform(k * n, torch.float32, -1, 1, seed=s).reshape(k, n) if k > 1 \
+            else drx.gen.uniform(n, torch.float32, -1, 1, seed=s)
+    xs = [vec(A.shape[1], 100 + i) for i, A in enumerate(mats)]
+    dys = [vec(A.shape[0], 200 + i) for i, A in enumerate(mats)]
+    ys = [torch.zeros_like(x) for x in xs]
+    dxs = [torch.zeros_like(d) for d in dys]
+    b = drx.Batch(forward=[(A, x, y) for A, x, y in zip(mats, xs, ys)],
+                  adjoint=[(A, d, dx) for A, d, dx in zip(mats, dys, dxs)], k=k)
+    b.set_tuning(1024, 0, 4)
+    b.run()
+    torch.cuda.synchronize()
+    for A, x, y, d, dx in zip(mats, xs, ys, dys, dxs):
+        assert torch.equal(y, A.matvec(x))
+        assert torch.equal(dx, A.rmatvec(d))
+        rp, ci, v = A.rowptr.cpu().numpy(), A.colind.cpu().numpy(), A.vals.cpu().numpy()
+        ys_, absy = orc.spmv(rp, ci, v, x.cpu().numpy())
+        assert componentwise_err(y.cpu().numpy(), ys_, absy) < 1e-5
+        dx_, absx = orc.spmv_t(rp, ci, v, d.cpu().numpy(), A.shape[1])
+        assert componentwise_err(dx.cpu().numpy(), dx_, absx) < 1e-5
+    b.close()
+
+
+def test_batch_graph_replays_and_autotune(orc, tmp_path):
+    """The captured graph is replayed several times with eager work and
+    synchronizations between replays (new x each time); every replay equals
+    the eager launch. The autotuned batch still matches the single calls."""
+    mats = _batch_mats(6, 11)
+    xs = [drx.gen.uniform(A.shape[1], torch.float32, -1, 1, seed=i) for i, A in enumerate(mats)]
+    ys = [torch.zeros(A.shape[0], device=DEV) for A in mats]
+    b = drx.Batch(forward=[(A, x, y) for A, x, y in zip(mats, xs, ys)])
+    t = b.autotune(iters=2, cache=str(tmp_path / "tune.json"), key="t6")
+    assert t["block_nnz"] in (256, 512, 1024, 2048) and t["wg_per_cu"] >= 1
+    t2 = b.autotune(cache=str(tmp_path / "tune.json"), key="t6")
+    assert t2.get("cached") and t2["block_nnz"] == t["block_nnz"]
+    b.graph()
+    for rep in range(4):
+        for i, x in enumerate(xs):
+            x.copy_(drx.gen.uniform(x.numel(), torch.float32, -1, 1, seed=50 + rep * 10 + i))
+        torch.cuda.synchronize()
+        b.graph_launch()
+        torch.cuda.synchronize()
+        got = [y.clone() for y in ys]
+        for y in ys:
+            y.zero_()
+        b.run()
+        torch.cuda.synchronize()
+        for g, y, A, x in zip(got, ys, mats, xs):
+            assert torch.equal(g, y)
+            rp, ci, v = A.rowptr.cpu().numpy(), A.colind.cpu().numpy(), A.vals.cpu().numpy()
+            ys_, absy = orc.spmv(rp, ci, v, x.cpu().numpy())
+            assert componentwise_err(g.cpu().numpy(), ys_, absy) < 1e-5
+    b.close()
