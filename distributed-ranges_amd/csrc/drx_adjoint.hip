@@ -61,6 +61,14 @@ __global__ void k_gather_rows(const TI *rowid, const TI *perm, int64_t nnz,
   if (p < nnz) rowind[p] = rowid[static_cast<int64_t>(perm[p])];
 }
 
+template <typename TI, typename TV>
+__global__ void k_permute_vals(const TI *perm, const TV *vals, int64_t nnz,
+                               TV *tvals) {
+  for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       p < nnz; p += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    tvals[p] = vals[static_cast<int64_t>(perm[p])];
+}
+
 static inline unsigned blocks_for(int64_t n, int tpb) {
   return static_cast<unsigned>((n + tpb - 1) / tpb);
 }
@@ -124,6 +132,7 @@ static void free_transpose(Transpose *t) {
   if (t->colptr) (void)hipFree(t->colptr);
   if (t->rowind) (void)hipFree(t->rowind);
   if (t->perm) (void)hipFree(t->perm);
+  if (t->tvals) (void)hipFree(t->tvals);
   delete t;
 }
 
@@ -172,6 +181,48 @@ int drx_transpose_arrays(drx_transpose t, const void **colptr,
   });
 }
 
+int drx_transpose_set_values(drx_transpose t, int vtype, const void *vals,
+                             drx_stream_t stream) {
+  return guarded([&] {
+    DRX_REQUIRE(t != nullptr, "transpose is null");
+    Transpose *tt = reinterpret_cast<Transpose *>(t);
+    const int64_t nnz = tt->plan->nnz;
+    DRX_REQUIRE(nnz == 0 || vals != nullptr, "vals is null");
+    DRX_REQUIRE(vtype >= DRX_F64 && vtype <= DRX_F16, "bad vtype");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (tt->tvals == nullptr || tt->tvtype != vtype) {
+      if (tt->tvals) (void)hipFree(tt->tvals);
+      tt->tvals = nullptr;
+      DRX_HIP(hipMalloc(&tt->tvals, std::max<int64_t>(nnz, 1) * dtype_size(vtype)));
+      tt->tvtype = vtype;
+    }
+    if (nnz == 0) return;
+    const dim3 g(std::min<unsigned>(blocks_for(nnz, 256), 1u << 20));
+    const bool i32 = tt->itype == DRX_I32;
+#define DRX_PV(TV)                                                             \
+  do {                                                                         \
+    if (i32)                                                                   \
+      hipLaunchKernelGGL((k_permute_vals<int32_t, TV>), g, dim3(256), 0, s,    \
+                         static_cast<const int32_t *>(tt->perm),               \
+                         static_cast<const TV *>(vals), nnz,                   \
+                         static_cast<TV *>(tt->tvals));                        \
+    else                                                                       \
+      hipLaunchKernelGGL((k_permute_vals<int64_t, TV>), g, dim3(256), 0, s,    \
+                         static_cast<const int64_t *>(tt->perm),               \
+                         static_cast<const TV *>(vals), nnz,                   \
+                         static_cast<TV *>(tt->tvals));                        \
+  } while (0)
+    if (vtype == DRX_F64)
+      DRX_PV(double);
+    else if (vtype == DRX_F32)
+      DRX_PV(float);
+    else
+      DRX_PV(uint16_t);
+#undef DRX_PV
+    DRX_HIP(hipGetLastError());
+  });
+}
+
 int drx_spmv_adjoint(drx_transpose t, int vtype, const void *vals, int dytype,
                      const void *dy, int64_t lddy, int dxtype, void *dx,
                      int64_t lddx, int64_t k, double alpha, double beta,
@@ -179,9 +230,16 @@ int drx_spmv_adjoint(drx_transpose t, int vtype, const void *vals, int dytype,
   return guarded([&] {
     DRX_REQUIRE(t != nullptr, "transpose is null");
     Transpose *tt = reinterpret_cast<Transpose *>(t);
-    run_spmv_perm(*tt->plan, vtype, tt->rowind, vals, tt->perm, dytype, dy,
-                  lddy, dxtype, dx, lddx, k, alpha, beta,
-                  static_cast<hipStream_t>(stream));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (vals == nullptr) {
+      DRX_REQUIRE(tt->tvals != nullptr && tt->tvtype == vtype,
+                  "vals is null and no values of this dtype were set on the transpose");
+      run_spmv(*tt->plan, vtype, tt->rowind, tt->tvals, dytype, dy, lddy, dxtype,
+               dx, lddx, k, alpha, beta, s);
+    } else {
+      run_spmv_perm(*tt->plan, vtype, tt->rowind, vals, tt->perm, dytype, dy,
+                    lddy, dxtype, dx, lddx, k, alpha, beta, s);
+    }
   });
 }
 

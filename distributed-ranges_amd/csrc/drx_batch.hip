@@ -32,6 +32,7 @@ struct BatchItemHost {
   const Plan *src = nullptr;  // the user's plan (forward) or the transpose's
   int mode = kPlain;
   const void *colind = nullptr, *vals = nullptr, *perm = nullptr;
+  const void *vals_override = nullptr;  // adjoint: the transpose's own values
   const void *x = nullptr;
   int64_t ldx = 0;
   void *y = nullptr;
@@ -79,13 +80,22 @@ __global__ __launch_bounds__(WG) void k_batch(const FwdArgs *items,
     else
       itb = mid;
   }
+  // The item's argument record is copied once per item change: read from
+  // global memory inside the block body it would be re-loaded after every
+  // store to y.
+  FwdArgs cur = items[it];
+  int mode = modes[it];
   for (; w < hi; w += nw) {
-    while (off[it + 1] <= w) it++;
+    if (off[it + 1] <= w) {
+      while (off[it + 1] <= w) it++;
+      cur = items[it];
+      mode = modes[it];
+    }
     const int64_t blk = w - off[it];
-    if (modes[it] == kPerm)
-      process_block<WG, CAP, kPerm, true, TV, TX, TY, TA, TI>(items[it], blk, sm);
+    if (mode == kPerm)
+      process_block<WG, CAP, kPerm, true, TV, TX, TY, TA, TI>(cur, blk, sm);
     else
-      process_block<WG, CAP, kPlain, true, TV, TX, TY, TA, TI>(items[it], blk, sm);
+      process_block<WG, CAP, kPlain, true, TV, TX, TY, TA, TI>(cur, blk, sm);
   }
 }
 
@@ -294,19 +304,27 @@ int drx_batch_create(drx_batch *out, int64_t nitems, const drx_batch_item *items
         } else {
           const Transpose *t = reinterpret_cast<const Transpose *>(u.transpose);
           h.src = t->plan;
-          h.mode = kPerm;
           h.colind = t->rowind;
-          h.perm = t->perm;
+          if (u.vals == nullptr) {  // the transpose's stored CSC-order values
+            DRX_REQUIRE(t->tvals != nullptr && t->tvtype == vtype,
+                        "adjoint item without vals needs drx_transpose_set_values");
+            h.mode = kPlain;
+            h.vals_override = t->tvals;
+          } else {
+            h.mode = kPerm;
+            h.perm = t->perm;
+          }
         }
         const Plan &p = *h.src;
         if (i == 0) b->itype = p.itype;
         DRX_REQUIRE(p.itype == b->itype, "batch items must share an index type");
-        DRX_REQUIRE(p.nnz == 0 || (h.colind && u.vals && u.x), "null operand in a batch item");
+        DRX_REQUIRE(p.nnz == 0 || (h.colind && (u.vals || h.vals_override) && u.x),
+                    "null operand in a batch item");
         DRX_REQUIRE(p.nrows == 0 || u.y, "null output in a batch item");
         DRX_REQUIRE(k == 1 || (u.ldx >= p.ncols && u.ldy >= p.nrows), "ldx/ldy too small");
         DRX_REQUIRE(static_cast<uint64_t>(std::max<int64_t>(p.ncols, 1)) * xs < (uint64_t(1) << 32),
                     "batch items need ncols * sizeof(x) < 4 GiB");
-        h.vals = u.vals;
+        h.vals = h.vals_override ? h.vals_override : u.vals;
         h.x = u.x;
         h.ldx = u.ldx > 0 ? u.ldx : p.ncols;
         h.y = u.y;

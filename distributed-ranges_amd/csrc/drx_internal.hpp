@@ -48,6 +48,8 @@ struct Transpose {
   void *colptr = nullptr;  // ncols + 1
   void *rowind = nullptr;  // nnz, row ids local to the share
   void *perm = nullptr;    // nnz, share position of each CSC entry
+  void *tvals = nullptr;   // nnz, values in CSC order (drx_transpose_set_values)
+  int tvtype = -1;         // dtype of tvals
   int itype = 0;
 };
 

@@ -48,6 +48,7 @@ SIGNATURES = {
     "drx_transpose_destroy": (c_int, [c_vp]),
     "drx_transpose_arrays": (c_int, [c_vp, ctypes.POINTER(c_vp),
                                      ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)]),
+    "drx_transpose_set_values": (c_int, [c_vp, c_int, c_vp, c_vp]),
     "drx_spmv_adjoint": (c_int, [c_vp, c_int, c_vp, c_int, c_vp, c_i64, c_int,
                                  c_vp, c_i64, c_i64, c_dbl, c_dbl, c_vp]),
     "drx_csr_dvals": (c_int, [c_vp, c_vp, c_int, c_vp, c_i64, c_int, c_vp,

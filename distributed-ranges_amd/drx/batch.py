@@ -32,7 +32,8 @@ class Batch:
             t = A.transpose(stream)
             _, lddy = _cols(dy, A.shape[0])
             _, lddx = _cols(dx, A.shape[1])
-            items[len(forward) + j] = BatchItem(None, t.handle.value, None, A.vals.data_ptr(),
+            vptr = None if A.stored_transposed_values(A.vals) else A.vals.data_ptr()
+            items[len(forward) + j] = BatchItem(None, t.handle.value, None, vptr,
                                                 dy.data_ptr(), lddy, dx.data_ptr(), lddx)
             keep += [A, dy, dx]
             ref = ref or (A.vals.dtype, dy.dtype, dx.dtype)

@@ -132,6 +132,7 @@ class CsrMatrix:
                          base=base, has_sentinel=has_sentinel,
                          block_nnz=block_nnz, stream=stream)
         self._t = None
+        self._tvals_key = None  # (tensor id, version) of the values stored in CSC order
 
     @property
     def vcode(self):
@@ -176,6 +177,20 @@ class CsrMatrix:
             self._t = Transpose(self.plan, self.colind, stream)
         return self._t
 
+    def set_transposed_values(self, vals=None, stream=None):
+        """Store `vals` (default: the matrix's) in CSC order in the transpose
+        (drx_transpose_set_values): later adjoints with the same, unchanged
+        tensor stream them instead of gathering through the permutation."""
+        vals = self.vals if vals is None else vals
+        t = self.transpose(stream)
+        check(lib().drx_transpose_set_values(t.handle, DTYPE_CODE[vals.dtype], _p(vals),
+                                             _s(stream)))
+        self._tvals_key = (id(vals), vals._version, vals.dtype)
+
+    def stored_transposed_values(self, vals):
+        """True when the transpose holds `vals` (same tensor, unchanged)."""
+        return self._tvals_key == (id(vals), vals._version, vals.dtype)
+
     def rmatvec(self, dy, dx=None, alpha=1.0, beta=0.0, vals=None, stream=None):
         """dx = alpha*A^T*dy + beta*dx."""
         vals = self.vals if vals is None else vals
@@ -186,7 +201,8 @@ class CsrMatrix:
             shape = (self.shape[1],) if dy.dim() == 1 else (k, self.shape[1])
             dx = torch.empty(shape, dtype=xdt, device=dy.device)
         _, lddx = _cols(dx, self.shape[1])
-        check(lib().drx_spmv_adjoint(t.handle, DTYPE_CODE[vals.dtype], _p(vals),
+        vptr = ctypes.c_void_p(0) if self.stored_transposed_values(vals) else _p(vals)
+        check(lib().drx_spmv_adjoint(t.handle, DTYPE_CODE[vals.dtype], vptr,
                                      DTYPE_CODE[dy.dtype], _p(dy), lddy,
                                      DTYPE_CODE[dx.dtype], _p(dx), lddx, k,
                                      float(alpha), float(beta), _s(stream)))

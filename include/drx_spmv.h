@@ -148,8 +148,17 @@ int drx_transpose_destroy(drx_transpose t);
 int drx_transpose_arrays(drx_transpose t, const void **colptr,
                          const void **rowind, const void **perm);
 
+/* Store the values in transposed (CSC) order inside t: one gather through
+ * perm. Adjoint calls given vals == NULL then stream these like a forward
+ * product instead of gathering vals[perm[e]] at random; call again when the
+ * values change. */
+int drx_transpose_set_values(drx_transpose t, int vtype, const void *vals,
+                             drx_stream_t stream);
+
 /* dx[:, j] = alpha * A^T dy[:, j] + beta * dx[:, j]. Same dtype combos as
- * drx_spmv with (vtype, dytype, dxtype). Deterministic. */
+ * drx_spmv with (vtype, dytype, dxtype). vals: the CSR-order values (read
+ * through the permutation), or NULL to use the values stored by
+ * drx_transpose_set_values. Deterministic. */
 int drx_spmv_adjoint(drx_transpose t, int vtype, const void *vals, int dytype,
                      const void *dy, int64_t lddy, int dxtype, void *dx,
                      int64_t lddx, int64_t k, double alpha, double beta,
@@ -187,7 +196,8 @@ typedef struct drx_batch_item {
   drx_plan plan;           /* forward item: the matrix's plan (else NULL) */
   drx_transpose transpose; /* adjoint item: its transposed structure (else NULL) */
   const void *colind;      /* forward: the matrix's colind (adjoint: unused) */
-  const void *vals;        /* the matrix's values, CSR order */
+  const void *vals;        /* the matrix's values, CSR order; NULL for an
+                              adjoint item whose transpose holds its values */
   const void *x;           /* forward: x; adjoint: dy */
   int64_t ldx;             /* <= 0: the operand's length */
   void *y;                 /* forward: y; adjoint: dx */
