@@ -344,7 +344,19 @@ def main():
     ap.add_argument("--forms", default="all", help="all | none | comma list")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 2 GiB device copy 5 times (the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        a = torch.ones(1 << 29, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read_per_launch": a.numel() * 4,
+                          "copy_bytes_written_per_launch": b.numel() * 4, "launches": 5}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build runs the kernel on one GPU's "
               "share only (see the sharding notes); --gpus %d is not supported."
