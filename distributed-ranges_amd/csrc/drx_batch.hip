@@ -12,6 +12,11 @@
 // Nothing is allocated or cleared during a run, so the launch can be
 // captured in a graph and replayed.
 //
+// With k >= 2 right-hand sides each item's x is first copied into row-major
+// chunks (one copy kernel over a job table, before the persistent kernel and
+// captured with it); the chunk buffers are allocated when the batch is
+// created.
+//
 // Tunables: block_nnz (entries per block, i.e. rows per workgroup), gshift
 // (the lane-group thresholds: a block of mean row length L uses G lanes per
 // row chosen from L >> gshift), wg_per_cu (persistent grid size). The
@@ -37,6 +42,16 @@ struct BatchItemHost {
   int64_t ldx = 0;
   void *y = nullptr;
   int64_t ldy = 0;
+  void *xr = nullptr;  // k >= 2: row-major chunks of x
+};
+
+// One x -> xr copy of a batch with k >= 2: threads [start, start + nchunk *
+// ncols) of the copy kernel.
+struct XrJob {
+  const void *x;
+  int64_t ldx, ncols;
+  void *xr;
+  int64_t start;
 };
 
 struct Batch {
@@ -56,6 +71,10 @@ struct Batch {
   int num_cus = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  bool mrhs = false;            // k >= 2 through row-major chunks of x
+  void *xr_all = nullptr;       // all items' chunks
+  XrJob *d_jobs = nullptr;
+  int64_t njobs = 0, xr_threads = 0;
 };
 
 template <int WG, int CAP, typename TV, typename TX, typename TY, typename TA,
@@ -105,12 +124,118 @@ static auto batch_kernel() {
   return k_batch<WG, CAP, TV, TX, TY, TA, TI>;
 }
 
+// k >= 2: the same walk over (item, block) with the multi-RHS block body.
+template <int WG, int CAP, int KP, typename TV, typename TX, typename TY,
+          typename TA, typename TI>
+__global__ __launch_bounds__(WG) void k_batch_mrhs(const FwdArgs *items,
+                                                   const int *modes,
+                                                   const int64_t *off,
+                                                   int64_t nitems, int64_t total) {
+  __shared__ MrhsSmem<WG, CAP, TA> sm;
+  const int64_t per_xcd = (total + kXcds - 1) / kXcds;
+  const int64_t xcd = blockIdx.x % kXcds;
+  const int64_t lw = blockIdx.x / kXcds, nw = gridDim.x / kXcds;
+  const int64_t lo = xcd * per_xcd;
+  const int64_t hi = lo + per_xcd < total ? lo + per_xcd : total;
+  int64_t w = lo + lw;
+  if (w >= hi) return;
+  int64_t it = 0, itb = nitems;
+  while (it + 1 < itb) {
+    const int64_t mid = (it + itb) / 2;
+    if (off[mid] <= w)
+      it = mid;
+    else
+      itb = mid;
+  }
+  FwdArgs cur = items[it];
+  int mode = modes[it];
+  for (; w < hi; w += nw) {
+    if (off[it + 1] <= w) {
+      while (off[it + 1] <= w) it++;
+      cur = items[it];
+      mode = modes[it];
+    }
+    const int64_t blk = w - off[it];
+    if (mode == kPerm)
+      process_block_mrhs<WG, CAP, KP, kPerm, TV, TX, TY, TA, TI>(cur, blk, sm);
+    else
+      process_block_mrhs<WG, CAP, KP, kPlain, TV, TX, TY, TA, TI>(cur, blk, sm);
+  }
+}
+
+// All items' x -> xr copies (layout of k_x_rowmajor), job by binary search.
+template <int KP, typename T>
+__global__ void k_batch_xr(const XrJob *jobs, int64_t njobs, int64_t total,
+                           int64_t k) {
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       t < total; t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    int64_t lo = 0, hi = njobs;  // last job with start <= t
+    while (lo + 1 < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if (jobs[mid].start <= t)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const XrJob jb = jobs[lo];
+    const int64_t u = t - jb.start;
+    const int64_t jc = u / jb.ncols, c = u - jc * jb.ncols;
+    const T *x = static_cast<const T *>(jb.x);
+    T buf[KP];
+#pragma unroll
+    for (int jj = 0; jj < KP; jj++) {
+      const int64_t j = jc * KP + jj;
+      buf[jj] = j < k ? x[j * jb.ldx + c] : T(0);
+    }
+    constexpr int BYTES = KP * static_cast<int>(sizeof(T));
+    T *dst = static_cast<T *>(jb.xr) + u * KP;
+    if constexpr (BYTES >= 16) {
+#pragma unroll
+      for (int i = 0; i < BYTES / 16; i++) {
+        uint4 v;
+        __builtin_memcpy(&v, reinterpret_cast<const char *>(buf) + 16 * i, 16);
+        reinterpret_cast<uint4 *>(dst)[i] = v;
+      }
+    } else if constexpr (BYTES == 8) {
+      uint2 v;
+      __builtin_memcpy(&v, buf, 8);
+      *reinterpret_cast<uint2 *>(dst) = v;
+    } else {
+      uint32_t v;
+      __builtin_memcpy(&v, buf, 4);
+      *reinterpret_cast<uint32_t *>(dst) = v;
+    }
+  }
+}
+
 // Resolve the kernel for the batch's types and window; returns its address
 // and workgroup size.
+template <int KP, typename TV, typename TX, typename TY, typename TA,
+          typename TI>
+static void pick_mrhs_kp(const Batch &b, const void **fn, int *wg) {
+  if (2 * b.block_nnz <= 2048) {
+    *fn = reinterpret_cast<const void *>(k_batch_mrhs<256, 2048, KP, TV, TX, TY, TA, TI>);
+    *wg = 256;
+  } else {
+    *fn = reinterpret_cast<const void *>(k_batch_mrhs<512, 8192, KP, TV, TX, TY, TA, TI>);
+    *wg = 512;
+  }
+}
+
 template <typename TV, typename TX, typename TY, typename TA, typename TI>
 static void pick_kernel_t(const Batch &b, const void **fn, int *wg) {
   const int64_t cap = 2 * b.block_nnz;
-  if (cap <= 1024) {
+  if (b.mrhs) {
+    const int kp = mrhs_kp(b.k, b.xtype);
+    if (kp == 2)
+      pick_mrhs_kp<2, TV, TX, TY, TA, TI>(b, fn, wg);
+    else if (kp == 4)
+      pick_mrhs_kp<4, TV, TX, TY, TA, TI>(b, fn, wg);
+    else if (kp == 8)
+      pick_mrhs_kp<8, TV, TX, TY, TA, TI>(b, fn, wg);
+    else
+      pick_mrhs_kp<16, TV, TX, TY, TA, TI>(b, fn, wg);
+  } else if (cap <= 1024) {
     *fn = reinterpret_cast<const void *>(batch_kernel<256, 1024, TV, TX, TY, TA, TI>());
     *wg = 256;
   } else if (cap <= 2048) {
@@ -141,6 +266,75 @@ static void pick_kernel(const Batch &b, const void **fn, int *wg) {
     pick_kernel_i<f16, f16, float, float>(b, fn, wg);
   else
     throw Error(DRX_ERR_UNSUPPORTED, "unsupported (vtype, xtype, ytype) for a batch");
+}
+
+static void free_chunks(Batch &b) {
+  if (b.xr_all) (void)hipFree(b.xr_all);
+  if (b.d_jobs) (void)hipFree(b.d_jobs);
+  b.xr_all = nullptr;
+  b.d_jobs = nullptr;
+}
+
+// k >= 2: one buffer for all items' row-major chunks and the copy jobs.
+static void make_chunks(Batch &b, hipStream_t s) {
+  const int64_t n = static_cast<int64_t>(b.items.size());
+  const int kp = mrhs_kp(b.k, b.xtype);
+  const int64_t nchunk = (b.k + kp - 1) / kp;
+  std::vector<size_t> at(n);
+  size_t bytes = 0;
+  for (int64_t i = 0; i < n; i++) {
+    at[i] = bytes;
+    bytes += (mrhs_xr_bytes(b.items[i].src->ncols, b.k, b.xtype) + 255) & ~size_t(255);
+  }
+  DRX_HIP(hipMalloc(&b.xr_all, std::max<size_t>(bytes, 256)));
+  std::vector<XrJob> jobs;
+  int64_t start = 0;
+  for (int64_t i = 0; i < n; i++) {
+    BatchItemHost &h = b.items[i];
+    h.xr = static_cast<char *>(b.xr_all) + at[i];
+    const int64_t nc = h.src->ncols;
+    if (nc == 0 || h.src->nnz == 0) continue;
+    jobs.push_back(XrJob{h.x, h.ldx, nc, h.xr, start});
+    start += nchunk * nc;
+  }
+  b.njobs = static_cast<int64_t>(jobs.size());
+  b.xr_threads = start;
+  DRX_HIP(hipMalloc(&b.d_jobs, std::max<int64_t>(b.njobs, 1) * sizeof(XrJob)));
+  if (b.njobs > 0) {
+    DRX_HIP(hipMemcpyAsync(b.d_jobs, jobs.data(), b.njobs * sizeof(XrJob),
+                           hipMemcpyHostToDevice, s));
+    DRX_HIP(hipStreamSynchronize(s));
+  }
+}
+
+static void launch_chunks(const Batch &b, hipStream_t s) {
+  if (b.xr_threads == 0) return;
+  const int kp = mrhs_kp(b.k, b.xtype);
+  const dim3 g(static_cast<unsigned>(std::min<int64_t>((b.xr_threads + 255) / 256, 1 << 16)));
+  const size_t xs = dtype_size(b.xtype);
+#define DRX_BXR(KP, T)                                                         \
+  hipLaunchKernelGGL((k_batch_xr<KP, T>), g, dim3(256), 0, s, b.d_jobs,        \
+                     b.njobs, b.xr_threads, b.k)
+#define DRX_BXR_T(T)                                                           \
+  do {                                                                         \
+    if (kp == 2)                                                               \
+      DRX_BXR(2, T);                                                           \
+    else if (kp == 4)                                                          \
+      DRX_BXR(4, T);                                                           \
+    else if (kp == 8)                                                          \
+      DRX_BXR(8, T);                                                           \
+    else                                                                       \
+      DRX_BXR(16, T);                                                          \
+  } while (0)
+  if (xs == 8)
+    DRX_BXR_T(uint64_t);
+  else if (xs == 4)
+    DRX_BXR_T(uint32_t);
+  else
+    DRX_BXR_T(uint16_t);
+#undef DRX_BXR_T
+#undef DRX_BXR
+  DRX_HIP(hipGetLastError());
 }
 
 static void free_plans(Batch &b) {
@@ -196,6 +390,8 @@ static void replan(Batch &b, int64_t block_nnz, hipStream_t s) {
     a.alpha = b.alpha;
     a.beta = b.beta;
     a.gshift = b.gshift;
+    a.xr = it.xr;
+    a.ncols = p->ncols;
     args[i] = a;
     modes[i] = it.mode;
     off[i + 1] = off[i] + (p->nrows > 0 ? p->nblocks : 0);
@@ -234,6 +430,7 @@ static void launch(Batch &b, hipStream_t s) {
   per_cu = std::max(1, std::min(per_cu, b.wg_per_cu));
   const int64_t grid = static_cast<int64_t>(b.num_cus) * per_cu;  // multiple of 8
   int64_t nitems = static_cast<int64_t>(b.items.size());
+  if (b.mrhs) launch_chunks(b, s);
   void *params[] = {&b.d_args, &b.d_mode, &b.d_off, &nitems, &b.total};
   DRX_HIP(hipLaunchKernel(fn, dim3(static_cast<unsigned>(grid)), dim3(wg), params, 0, s));
 }
@@ -283,6 +480,7 @@ int drx_batch_create(drx_batch *out, int64_t nitems, const drx_batch_item *items
       b->xtype = xtype;
       b->ytype = ytype;
       b->k = k;
+      b->mrhs = k >= 2 && tuned_mrhs() != 0;
       b->alpha = alpha;
       b->beta = beta;
       int dev = 0;
@@ -334,11 +532,13 @@ int drx_batch_create(drx_batch *out, int64_t nitems, const drx_batch_item *items
       const void *fn;
       int wg;
       pick_kernel(*b, &fn, &wg);  // validates the type combination
+      if (b->mrhs) make_chunks(*b, s);
       const int64_t bn = tuned_block_nnz() > 0 ? tuned_block_nnz() : 1024;
       replan(*b, bn, s);
     } catch (...) {
       drop_graph(*b);
       free_plans(*b);
+      free_chunks(*b);
       delete b;
       throw;
     }
@@ -352,6 +552,7 @@ int drx_batch_destroy(drx_batch batch) {
     Batch *b = reinterpret_cast<Batch *>(batch);
     drop_graph(*b);
     free_plans(*b);
+    free_chunks(*b);
     delete b;
   });
 }
@@ -435,7 +636,8 @@ int drx_batch_autotune(drx_batch batch, int iters, char *json, int64_t json_cap,
     int best_g = 0, best_w = b.wg_per_cu;
     for (int64_t bn : {256, 512, 1024, 2048}) {
       replan(b, bn, s);
-      for (int g : {0, 1}) {
+      for (int g : {0, 1, 2}) {
+        if (g == 2 && !b.mrhs) continue;
         set_gshift(b, g, s);
         for (int w : {2, 4, 8, 16}) {
           b.wg_per_cu = w;

@@ -45,6 +45,8 @@ struct FwdArgs {
   int64_t ldy, k;
   double alpha, beta;
   int gshift;  // lane-group choice uses (mean row length >> gshift)
+  const void *xr;  // k >= 2: x in row-major chunks of KP columns
+  int64_t ncols;   // columns of the share (rows of one xr chunk)
 };
 
 template <int WG, typename TA>
@@ -149,12 +151,30 @@ template <int WG, int CAP, bool ROWMAP, typename TA> struct FwdSmem {
   TA red[WG / 64];
 };
 
+// Storage value from its raw bits (2- or 4-byte types).
+template <typename TV, typename U> __device__ __forceinline__ TV bits_to(U u) {
+  if constexpr (sizeof(TV) == 2)
+    return __builtin_bit_cast(TV, static_cast<uint16_t>(u));
+  else
+    return __builtin_bit_cast(TV, static_cast<uint32_t>(u));
+}
+
 // One row block `blk` of the share described by `a` (all lanes of the
 // workgroup call it together).
+//
+// VW = 2 (4-byte or smaller values, 32-bit indices, plain or row-map mode):
+// each lane streams pairs of consecutive entries with one 8-byte colind load
+// and one 4- or 8-byte vals load, from the pair-aligned entry below the
+// window start (the caller guarantees pair-aligned vals / colind pointers).
+// The window's last pair may extend one entry past the window (and past the
+// arrays' last entry) inside that same aligned pair; such entries are read
+// but never used.
 template <int WG, int CAP, int MODE, bool BUFX, typename TV, typename TX,
-          typename TY, typename TA, typename TI>
+          typename TY, typename TA, typename TI, int VW = 1>
 __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
                                               FwdSmem<WG, CAP, MODE == kRowMap, TA> &sm) {
+  static_assert(VW == 1 || (VW == 2 && MODE != kPerm && sizeof(TI) == 4 && sizeof(TV) <= 4),
+                "pairs of entries: plain or row-map mode, 32-bit indices, <= 4-byte values");
   constexpr int T = CAP / WG;
   constexpr int ROWCAP = CAP / 2;
   TA *prod = sm.prod;
@@ -178,11 +198,49 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
     const int n_s = static_cast<int>(b.es - b.e0);
     const int64_t nr = (b.es == e1 ? r1 : r1 - 1) - b.r0;
     const bool rows_lds = nr < ROWCAP;
+    // entry (relative to e0) held in slot u of a lane
+    const int sh = VW > 1 ? static_cast<int>(b.e0 & (VW - 1)) : 0;
+    auto eidx = [&](int u) -> int {
+      if constexpr (VW == 1)
+        return tid + u * WG;
+      else
+        return (tid + (u / VW) * WG) * VW + (u % VW) - sh;
+    };
 
     // Stream the window (reads past n_s return 0 and touch no memory).
     TA v[T];
     uint32_t c[T];
-    {
+    // VW = 2: the lanes' pairs cover entries [-sh, CAP - sh); a full window
+    // (n_s == CAP) starting at an odd entry leaves entry CAP - 1 to lane 0.
+    const bool tail = VW == 2 && n_s + sh > CAP;
+    TA v_tail = 0;
+    uint32_t c_tail = 0;
+    if constexpr (VW == 2) {
+      if (tail && tid == 0) {
+        v_tail = to_acc<TA>(vals[b.e0 + CAP - 1]);
+        c_tail = col32(colind[b.e0 + CAP - 1]);
+      }
+      const int64_t ea = b.e0 - sh;
+      const uint32_t npair = static_cast<uint32_t>(n_s + sh + 1) / 2;
+      const rsrc_t rv = make_rsrc(vals + ea, npair * 2 * sizeof(TV));
+      const rsrc_t rc = make_rsrc(colind + ea, npair * 2 * sizeof(TI));
+#pragma unroll
+      for (int p = 0; p < T / 2; p++) {
+        const uint32_t pi = tid + p * WG;
+        const uint64_t cw = buf_load<uint64_t, kNT>(rc, pi * 8);
+        c[2 * p] = static_cast<uint32_t>(cw);
+        c[2 * p + 1] = static_cast<uint32_t>(cw >> 32);
+        if constexpr (sizeof(TV) == 4) {
+          const uint64_t w = buf_load<uint64_t, kNT>(rv, pi * 8);
+          v[2 * p] = to_acc<TA>(bits_to<TV>(static_cast<uint32_t>(w)));
+          v[2 * p + 1] = to_acc<TA>(bits_to<TV>(static_cast<uint32_t>(w >> 32)));
+        } else {
+          const uint32_t w = buf_load<uint32_t, kNT>(rv, pi * 4);
+          v[2 * p] = to_acc<TA>(bits_to<TV>(w & 0xffffu));
+          v[2 * p + 1] = to_acc<TA>(bits_to<TV>(w >> 16));
+        }
+      }
+    } else {
       const rsrc_t rc = make_rsrc(colind + b.e0, n_s * sizeof(TI));
       if constexpr (MODE == kPerm) {
         const rsrc_t rp = make_rsrc(perm + b.e0, n_s * sizeof(TI));
@@ -223,15 +281,18 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
         const rsrc_t rx = make_rsrc(xj, a.xbytes);
 #pragma unroll
         for (int u = 0; u < T; u++) {
-          const int idx = tid + u * WG;
-          if (idx < n_s) prod[idx] = v[u] * to_acc<TA>(buf_load<TX>(rx, c[u] * sizeof(TX)));
+          const int idx = eidx(u);
+          if (idx >= 0 && idx < n_s)
+            prod[idx] = v[u] * to_acc<TA>(buf_load<TX>(rx, c[u] * sizeof(TX)));
         }
+        if (tail && tid == 0) prod[CAP - 1] = v_tail * to_acc<TA>(buf_load<TX>(rx, c_tail * sizeof(TX)));
       } else {
 #pragma unroll
         for (int u = 0; u < T; u++) {
-          const int idx = tid + u * WG;
-          if (idx < n_s) prod[idx] = v[u] * to_acc<TA>(xj[c[u]]);
+          const int idx = eidx(u);
+          if (idx >= 0 && idx < n_s) prod[idx] = v[u] * to_acc<TA>(xj[c[u]]);
         }
+        if (tail && tid == 0) prod[CAP - 1] = v_tail * to_acc<TA>(xj[c_tail]);
       }
       __syncthreads();
       reduce_rows_any<WG, MODE, TA, TY, TI>(n_s, prod, rp_s, map_s,
@@ -284,14 +345,313 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
 }
 
 template <int WG, int CAP, int MODE, bool BUFX, typename TV, typename TX,
-          typename TY, typename TA, typename TI>
+          typename TY, typename TA, typename TI, int VW = 1>
 __global__ __launch_bounds__(WG) void k_spmv_fwd(FwdArgs a) {
   __shared__ FwdSmem<WG, CAP, MODE == kRowMap, TA> sm;
   for (int64_t wi = blockIdx.x; wi < a.per_xcd * kXcds; wi += a.grid) {
     const int64_t blk = map_block(wi, a.per_xcd);
     if (blk >= a.nblocks) continue;
-    process_block<WG, CAP, MODE, BUFX, TV, TX, TY, TA, TI>(a, blk, sm);
+    process_block<WG, CAP, MODE, BUFX, TV, TX, TY, TA, TI, VW>(a, blk, sm);
   }
 }
+
+}  // namespace drx
+
+// ---------------------------------------------------------------------------
+// Several right-hand sides (k >= 2).
+//
+// x is first copied into row-major chunks of KP columns (KP = 2, 4, 8 or 16
+// by k; at most 8 for 8-byte x, whose 16 running sums would cost
+// occupancy),
+//   xr[(chunk * ncols + c) * KP + jj] = x[(chunk * KP + jj) * ldx + c]
+// (zero padded past k), so that an entry gathers one contiguous KP-vector
+// (16-128 bytes) instead of k values from k separate columns. The window's
+// values and column indices are staged in LDS; groups of G lanes then walk
+// their rows, each lane holding KP running sums, and the G lanes of a row
+// write its KP outputs (column jj by lane jj mod G).
+// ---------------------------------------------------------------------------
+namespace drx {
+
+template <int WG, int CAP, typename TA> struct MrhsSmem {
+  TA v[CAP];
+  uint32_t c[CAP];
+  int rp_s[CAP / 2 + 1];
+  TA red[16 * (WG / 64)];
+};
+
+// KP consecutive x values (one xr row) converted to the accumulator type.
+template <int KP, typename TX, typename TA>
+__device__ __forceinline__ void load_xrow(const TX *p, TA (&xv)[KP]) {
+  constexpr int BYTES = KP * static_cast<int>(sizeof(TX));
+  static_assert(BYTES % 4 == 0, "an xr row is a multiple of 4 bytes");
+  if constexpr (BYTES >= 16) {
+    constexpr int PER = 16 / static_cast<int>(sizeof(TX));
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+#pragma unroll
+    for (int i = 0; i < BYTES / 16; i++) {
+      const uint4 w = q[i];
+      TX t[PER];
+      __builtin_memcpy(t, &w, 16);
+#pragma unroll
+      for (int s = 0; s < PER; s++) xv[i * PER + s] = to_acc<TA>(t[s]);
+    }
+  } else if constexpr (BYTES == 8) {
+    const uint2 w = *reinterpret_cast<const uint2 *>(p);
+    TX t[KP];
+    __builtin_memcpy(t, &w, 8);
+#pragma unroll
+    for (int s = 0; s < KP; s++) xv[s] = to_acc<TA>(t[s]);
+  } else {
+    const uint32_t w = *reinterpret_cast<const uint32_t *>(p);
+    TX t[KP];
+    __builtin_memcpy(t, &w, 4);
+#pragma unroll
+    for (int s = 0; s < KP; s++) xv[s] = to_acc<TA>(t[s]);
+  }
+}
+
+template <int WG, int G, int KP, typename TX, typename TY, typename TA,
+          typename TI>
+__device__ __forceinline__ void reduce_rows_mrhs(
+    const TA *v_s, const uint32_t *c_s, const int *rp_s, bool rows_lds,
+    const FwdArgs &a, int64_t r0, int64_t e0, int64_t nr, const TX *xrc,
+    int64_t j0, int kk, TA alpha, TA beta) {
+  const TI *rowptr = static_cast<const TI *>(a.rowptr);
+  TY *y = static_cast<TY *>(a.y);
+  const int lane = threadIdx.x & (G - 1);
+  for (int64_t i = threadIdx.x / G; i < nr; i += WG / G) {
+    int s, e;
+    if (rows_lds) {
+      s = rp_s[i];
+      e = rp_s[i + 1];
+    } else {
+      s = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r0 + i) - e0);
+      e = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r0 + i + 1) - e0);
+    }
+    TA acc[KP];
+#pragma unroll
+    for (int jj = 0; jj < KP; jj++) acc[jj] = 0;
+    int q = s + lane;
+    for (; q + G < e; q += 2 * G) {
+      TA x0[KP], x1[KP];
+      load_xrow<KP>(xrc + static_cast<size_t>(c_s[q]) * KP, x0);
+      load_xrow<KP>(xrc + static_cast<size_t>(c_s[q + G]) * KP, x1);
+      const TA v0 = v_s[q], v1 = v_s[q + G];
+#pragma unroll
+      for (int jj = 0; jj < KP; jj++) {
+        acc[jj] += v0 * x0[jj];
+        acc[jj] += v1 * x1[jj];
+      }
+    }
+    if (q < e) {
+      TA x0[KP];
+      load_xrow<KP>(xrc + static_cast<size_t>(c_s[q]) * KP, x0);
+      const TA v0 = v_s[q];
+#pragma unroll
+      for (int jj = 0; jj < KP; jj++) acc[jj] += v0 * x0[jj];
+    }
+#pragma unroll
+    for (int jj = 0; jj < KP; jj++) acc[jj] = group_sum<G>(acc[jj]);
+    const int64_t row = r0 + i;
+#pragma unroll
+    for (int jj = 0; jj < KP; jj++) {
+      if (jj < kk && (jj & (G - 1)) == lane) {
+        TY *yp = y + (j0 + jj) * a.ldy + row;
+        TA out = alpha * acc[jj];
+        if (beta != TA(0)) out += beta * to_acc<TA>(*yp);
+        *yp = from_acc<TY>(out);
+      }
+    }
+  }
+}
+
+template <int WG, int KP, typename TX, typename TY, typename TA, typename TI>
+__device__ __forceinline__ void reduce_rows_mrhs_any(
+    int n_s, const TA *v_s, const uint32_t *c_s, const int *rp_s,
+    bool rows_lds, const FwdArgs &a, int64_t r0, int64_t e0, int64_t nr,
+    const TX *xrc, int64_t j0, int kk, TA alpha, TA beta) {
+  if (nr <= 0) return;
+  const int64_t avg = (n_s / nr) >> a.gshift;
+#define DRX_RM(G) reduce_rows_mrhs<WG, G, KP, TX, TY, TA, TI>(v_s, c_s, rp_s, rows_lds, a, r0, e0, nr, xrc, j0, kk, alpha, beta)
+  if (avg <= 3)
+    DRX_RM(1);
+  else if (avg <= 6)
+    DRX_RM(2);
+  else if (avg <= 12)
+    DRX_RM(4);
+  else if (avg <= 24)
+    DRX_RM(8);
+  else if (avg <= 48)
+    DRX_RM(16);
+  else if (avg <= 96)
+    DRX_RM(32);
+  else
+    DRX_RM(64);
+#undef DRX_RM
+}
+
+// One row block for k >= 2 right-hand sides, x read from a.xr (all lanes of
+// the workgroup call it together). MODE is kPlain or kPerm.
+template <int WG, int CAP, int KP, int MODE, typename TV, typename TX,
+          typename TY, typename TA, typename TI>
+__device__ __forceinline__ void process_block_mrhs(const FwdArgs &a,
+                                                   int64_t blk,
+                                                   MrhsSmem<WG, CAP, TA> &sm) {
+  static_assert(MODE == kPlain || MODE == kPerm, "row maps use k = 1 panels");
+  constexpr int T = CAP / WG;
+  constexpr int ROWCAP = CAP / 2;
+  constexpr int NW = WG / 64;
+  const int tid = threadIdx.x;
+  const TI *rowptr = static_cast<const TI *>(a.rowptr);
+  const TI *colind = static_cast<const TI *>(a.colind);
+  const TV *vals = static_cast<const TV *>(a.vals);
+  const TI *perm = static_cast<const TI *>(a.perm);
+  const TX *xr = static_cast<const TX *>(a.xr);
+  TY *y = static_cast<TY *>(a.y);
+  const TA alpha = static_cast<TA>(a.alpha), beta = static_cast<TA>(a.beta);
+
+  const BlockRec b = a.rec[blk];
+  const int64_t r1 = a.rec[blk + 1].r0, e1 = a.rec[blk + 1].e0;
+  if (b.r0 >= r1) return;
+  const int n_s = static_cast<int>(b.es - b.e0);
+  const int64_t nr = (b.es == e1 ? r1 : r1 - 1) - b.r0;
+  const bool rows_lds = nr < ROWCAP;
+
+  // Stage the window in LDS (reads past n_s return 0 and touch no memory).
+  {
+    TA v[T];
+    uint32_t c[T];
+    const rsrc_t rc = make_rsrc(colind + b.e0, n_s * sizeof(TI));
+    if constexpr (MODE == kPerm) {
+      const rsrc_t rp = make_rsrc(perm + b.e0, n_s * sizeof(TI));
+#pragma unroll
+      for (int u = 0; u < T; u++) {
+        const uint32_t idx = tid + u * WG;
+        c[u] = col32(buf_load<TI, kNT>(rc, idx * sizeof(TI)));
+        const int64_t pe = static_cast<int64_t>(buf_load<TI, kNT>(rp, idx * sizeof(TI)));
+        v[u] = idx < static_cast<uint32_t>(n_s) ? to_acc<TA>(vals[pe]) : TA(0);
+      }
+    } else {
+      const rsrc_t rv = make_rsrc(vals + b.e0, n_s * sizeof(TV));
+#pragma unroll
+      for (int u = 0; u < T; u++) {
+        const uint32_t idx = tid + u * WG;
+        v[u] = to_acc<TA>(buf_load<TV, kNT>(rv, idx * sizeof(TV)));
+        c[u] = col32(buf_load<TI, kNT>(rc, idx * sizeof(TI)));
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < T; u++) {
+      const int idx = tid + u * WG;
+      if (idx < n_s) {
+        sm.v[idx] = v[u];
+        sm.c[idx] = c[u];
+      }
+    }
+  }
+  if (rows_lds) {
+    for (int i = tid; i <= nr; i += WG)
+      sm.rp_s[i] = static_cast<int>(
+          row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, b.r0 + i) - b.e0);
+  }
+  __syncthreads();
+
+  const int64_t nchunk = (a.k + KP - 1) / KP;
+  for (int64_t jc = 0; jc < nchunk; jc++) {
+    const int kk = static_cast<int>(a.k - jc * KP < KP ? a.k - jc * KP : KP);
+    reduce_rows_mrhs_any<WG, KP, TX, TY, TA, TI>(
+        n_s, sm.v, sm.c, sm.rp_s, rows_lds, a, b.r0, b.e0, nr,
+        xr + jc * a.ncols * KP, jc * KP, kk, alpha, beta);
+  }
+
+  if (b.es < e1) {
+    // Long last row: the whole workgroup, strided coalesced loads.
+    const int64_t last = r1 - 1;
+    for (int64_t jc = 0; jc < nchunk; jc++) {
+      const int kk = static_cast<int>(a.k - jc * KP < KP ? a.k - jc * KP : KP);
+      const TX *xrc = xr + jc * a.ncols * KP;
+      TA acc[KP];
+#pragma unroll
+      for (int jj = 0; jj < KP; jj++) acc[jj] = 0;
+      for (int64_t q = b.es + tid; q < e1; q += WG) {
+        const TA w = to_acc<TA>(long_val<MODE>(vals, perm, q));
+        TA xv[KP];
+        load_xrow<KP>(xrc + static_cast<size_t>(ld_stream(colind + q)) * KP, xv);
+#pragma unroll
+        for (int jj = 0; jj < KP; jj++) acc[jj] += w * xv[jj];
+      }
+#pragma unroll
+      for (int jj = 0; jj < KP; jj++) acc[jj] = group_sum<64>(acc[jj]);
+      __syncthreads();
+      if ((tid & 63) == 0) {
+#pragma unroll
+        for (int jj = 0; jj < KP; jj++) sm.red[jj * NW + (tid >> 6)] = acc[jj];
+      }
+      __syncthreads();
+      if (tid < kk) {
+        TA s = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) s += sm.red[tid * NW + w];
+        TY *yp = y + (jc * KP + tid) * a.ldy + last;
+        TA out = alpha * s;
+        if (beta != TA(0)) out += beta * to_acc<TA>(*yp);
+        *yp = from_acc<TY>(out);
+      }
+    }
+  }
+  __syncthreads();  // the next block overwrites the LDS window
+}
+
+template <int WG, int CAP, int KP, int MODE, typename TV, typename TX,
+          typename TY, typename TA, typename TI>
+__global__ __launch_bounds__(WG) void k_spmv_mrhs(FwdArgs a) {
+  __shared__ MrhsSmem<WG, CAP, TA> sm;
+  for (int64_t wi = blockIdx.x; wi < a.per_xcd * kXcds; wi += a.grid) {
+    const int64_t blk = map_block(wi, a.per_xcd);
+    if (blk >= a.nblocks) continue;
+    process_block_mrhs<WG, CAP, KP, MODE, TV, TX, TY, TA, TI>(a, blk, sm);
+  }
+}
+
+// Row-major chunks of x (layout above); one thread per (chunk, column).
+template <int KP, typename TX>
+__global__ void k_x_rowmajor(const TX *x, int64_t ldx, int64_t ncols,
+                             int64_t k, TX *xr) {
+  const int64_t nchunk = (k + KP - 1) / KP;
+  const int64_t total = nchunk * ncols;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       t < total; t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t jc = t / ncols, c = t - jc * ncols;
+    TX buf[KP];
+#pragma unroll
+    for (int jj = 0; jj < KP; jj++) {
+      const int64_t j = jc * KP + jj;
+      buf[jj] = j < k ? x[j * ldx + c] : TX{};
+    }
+    constexpr int BYTES = KP * static_cast<int>(sizeof(TX));
+    if constexpr (BYTES >= 16) {
+      uint4 *dst = reinterpret_cast<uint4 *>(xr + t * KP);
+#pragma unroll
+      for (int i = 0; i < BYTES / 16; i++) {
+        uint4 w;
+        __builtin_memcpy(&w, reinterpret_cast<const char *>(buf) + 16 * i, 16);
+        dst[i] = w;
+      }
+    } else if constexpr (BYTES == 8) {
+      uint2 w;
+      __builtin_memcpy(&w, buf, 8);
+      *reinterpret_cast<uint2 *>(xr + t * KP) = w;
+    } else {
+      uint32_t w;
+      __builtin_memcpy(&w, buf, 4);
+      *reinterpret_cast<uint32_t *>(xr + t * KP) = w;
+    }
+  }
+}
+
+// Host side (drx_mrhs.hip): copy x into row-major chunks and launch the
+// multi-RHS kernel for a plain (mode kPlain) or permuted (kPerm) share.
+void run_mrhs(const Plan &p, FwdArgs a, int mode, int vtype, int xtype,
+              int ytype, hipStream_t s);
 
 }  // namespace drx

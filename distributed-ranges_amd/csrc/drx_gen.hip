@@ -19,6 +19,15 @@ namespace drx {
 // DRX_BLOCK_NNZ seeds block_nnz when nothing was set through the API.
 // ---------------------------------------------------------------------------
 static std::atomic<int64_t> g_block_nnz{0};
+static std::atomic<int> g_mrhs{1};
+static std::atomic<int> g_mrhs_gshift{1};
+static std::atomic<int> g_vec{0};
+static std::atomic<int64_t> g_grid_cap{0};
+
+int tuned_mrhs() { return g_mrhs.load(); }
+int tuned_mrhs_gshift() { return g_mrhs_gshift.load(); }
+int tuned_vec() { return g_vec.load(); }
+int64_t tuned_grid_cap() { return g_grid_cap.load(); }
 
 int64_t tuned_block_nnz() {
   int64_t v = g_block_nnz.load();
@@ -72,6 +81,31 @@ __global__ void k_normal(T *out, int64_t n, uint64_t seed) {
                       (1.0 / 9007199254740992.0);
     const double u2 = u01(rnd(seed, 2 * i + 1));
     out[i] = store_as<T>(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+  }
+}
+
+// out[i] = x[idx[i]]: streamed (non-temporal) indices, buffer gathers of x
+// with 32-bit offsets, 4 independent gathers per lane in flight -- the
+// memory pattern of a sparse product's gathers without its reduction.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather(const int32_t *idx, int64_t n,
+                                                const void *x, uint32_t xbytes, T *out) {
+  const rsrc_t rx = make_rsrc(x, xbytes);
+  const int64_t step = static_cast<int64_t>(gridDim.x) * 1024;
+  for (int64_t base = static_cast<int64_t>(blockIdx.x) * 1024; base < n; base += step) {
+    int64_t e[4];
+    uint32_t c[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      e[u] = base + threadIdx.x + u * 256;
+      c[u] = e[u] < n ? static_cast<uint32_t>(__builtin_nontemporal_load(idx + e[u])) : 0u;
+    }
+    T v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) v[u] = buf_load<T>(rx, c[u] * static_cast<uint32_t>(sizeof(T)));
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (e[u] < n) out[e[u]] = v[u];
   }
 }
 
@@ -297,6 +331,18 @@ int drx_tune_set(const char *key, int64_t value) {
                                  (value & (value - 1)) == 0),
                   "block_nnz must be 0 or a power of two in [64, 4096]");
       g_block_nnz.store(value);
+    } else if (std::strcmp(key, "mrhs") == 0) {
+      DRX_REQUIRE(value == 0 || value == 1, "mrhs must be 0 or 1");
+      g_mrhs.store(static_cast<int>(value));
+    } else if (std::strcmp(key, "mrhs_gshift") == 0) {
+      DRX_REQUIRE(value >= 0 && value <= 3, "mrhs_gshift must be in [0, 3]");
+      g_mrhs_gshift.store(static_cast<int>(value));
+    } else if (std::strcmp(key, "vec") == 0) {
+      DRX_REQUIRE(value >= 0 && value <= 2, "vec must be 0, 1 or 2");
+      g_vec.store(static_cast<int>(value));
+    } else if (std::strcmp(key, "grid_cap") == 0) {
+      DRX_REQUIRE(value >= 0 && value <= (int64_t(1) << 30), "grid_cap out of range");
+      g_grid_cap.store(value);
     } else {
       throw Error(DRX_ERR_INVALID, std::string("unknown tuning key: ") + key);
     }
@@ -308,6 +354,14 @@ int drx_tune_get(const char *key, int64_t *value) {
     DRX_REQUIRE(key != nullptr && value != nullptr, "null argument");
     if (std::strcmp(key, "block_nnz") == 0)
       *value = default_block_nnz(DRX_I32);
+    else if (std::strcmp(key, "mrhs") == 0)
+      *value = g_mrhs.load();
+    else if (std::strcmp(key, "mrhs_gshift") == 0)
+      *value = g_mrhs_gshift.load();
+    else if (std::strcmp(key, "vec") == 0)
+      *value = g_vec.load();
+    else if (std::strcmp(key, "grid_cap") == 0)
+      *value = g_grid_cap.load();
     else
       throw Error(DRX_ERR_INVALID, std::string("unknown tuning key: ") + key);
   });
@@ -526,6 +580,28 @@ int drx_gen_band(int64_t n, int64_t lo, int64_t hi, int itype, void *rowptr,
       int st = drx_gen_normal(vtype, vals, nnz, seed, stream);
       if (st != DRX_OK) throw Error(st, drx_last_error());
     }
+  });
+}
+
+int drx_gather(const int32_t *idx, int64_t n, int xtype, const void *x,
+               int64_t nx, void *out, drx_stream_t stream) {
+  return guarded([&] {
+    DRX_REQUIRE(n >= 0 && nx >= 0 && (n == 0 || (idx && x && out)), "bad argument");
+    DRX_REQUIRE(xtype >= DRX_F64 && xtype <= DRX_F16, "bad xtype");
+    DRX_REQUIRE(static_cast<uint64_t>(std::max<int64_t>(nx, 1)) * dtype_size(xtype) <
+                    (uint64_t(1) << 32),
+                "x must be below 4 GiB");
+    if (n == 0) return;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint32_t xb = static_cast<uint32_t>(std::max<int64_t>(nx, 1) * dtype_size(xtype));
+    const int64_t per = 1024;  // indices per workgroup (4 per lane)
+    const dim3 g(static_cast<unsigned>(std::min<int64_t>((n + per - 1) / per, int64_t(1) << 22)));
+    switch (dtype_size(xtype)) {
+      case 8: hipLaunchKernelGGL(k_gather<uint64_t>, g, dim3(256), 0, s, idx, n, x, xb, static_cast<uint64_t *>(out)); break;
+      case 4: hipLaunchKernelGGL(k_gather<uint32_t>, g, dim3(256), 0, s, idx, n, x, xb, static_cast<uint32_t *>(out)); break;
+      default: hipLaunchKernelGGL(k_gather<uint16_t>, g, dim3(256), 0, s, idx, n, x, xb, static_cast<uint16_t *>(out)); break;
+    }
+    DRX_HIP(hipGetLastError());
   });
 }
 

@@ -55,6 +55,27 @@ struct Transpose {
 
 int64_t tuned_block_nnz();
 int64_t default_block_nnz(int itype);
+// k >= 2: 1 = gather from row-major chunks of x (default), 0 = gather from
+// the k columns of x directly; lane-group shift of the row-major kernel.
+int tuned_mrhs();
+int tuned_mrhs_gshift();
+// Entries per lane of the streaming loads: 0 = by type and mode, 1, or 2
+// where eligible;
+// the most workgroups of a per-share launch (0 = one per row block, up to
+// 2^22), which then loop over blocks.
+int tuned_vec();
+int64_t tuned_grid_cap();
+
+// Stream-ordered scratch from the library's per-device pool.
+void *scratch_alloc(size_t bytes, hipStream_t s);
+void scratch_free(void *p, hipStream_t s);
+
+// Row-major chunks of x for k >= 2 right-hand sides: chunk width KP and the
+// bytes of all chunks; the copy kernel.
+int mrhs_kp(int64_t k, int xtype);
+size_t mrhs_xr_bytes(int64_t ncols, int64_t k, int xtype);
+void launch_x_rowmajor(int xtype, const void *x, int64_t ldx, int64_t ncols,
+                       int64_t k, void *xr, hipStream_t s);
 
 Plan *make_plan(int itype, const void *rowptr, int64_t base, int64_t nrows,
                 int64_t ncols, int64_t nnz, int sent, int64_t block_nnz,
@@ -75,6 +96,10 @@ void run_spmv_rowmap(const Plan &p, int vtype, const void *colind,
                      const void *x, int64_t ldx, int ytype, void *y,
                      int64_t ldy, int64_t k, double alpha, hipStream_t s);
 void free_panels(PanelSet *ps);
+// A plan with panels is called with the colind/vals it was built from (or
+// last updated to); throws otherwise.
+void check_panel_operands(const Plan &p, int vtype, const void *colind,
+                          const void *vals);
 
 // Forward kernel reading vals through a permutation (vals[perm[e]]), used for
 // the transposed structure.

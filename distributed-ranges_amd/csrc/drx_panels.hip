@@ -259,15 +259,21 @@ __global__ void k_scale_y(TY *y, int64_t nrows, int64_t ldy, int64_t k,
   }
 }
 
-void run_spmv_panels(const Plan &p, int vtype, const void *colind,
-                     const void *vals, int xtype, const void *x, int64_t ldx,
-                     int ytype, void *y, int64_t ldy, int64_t k, double alpha,
-                     double beta, hipStream_t s) {
+void check_panel_operands(const Plan &p, int vtype, const void *colind,
+                          const void *vals) {
   const PanelSet &ps = *p.panels;
   DRX_REQUIRE(vtype == ps.vtype, "vtype differs from the panel layout's");
   DRX_REQUIRE(vals == ps.src_vals && colind == ps.src_colind,
               "the panel layout holds a copy of the colind/vals it was built "
               "from: pass those pointers (drx_plan_update_values re-packs new values)");
+}
+
+void run_spmv_panels(const Plan &p, int vtype, const void *colind,
+                     const void *vals, int xtype, const void *x, int64_t ldx,
+                     int ytype, void *y, int64_t ldy, int64_t k, double alpha,
+                     double beta, hipStream_t s) {
+  const PanelSet &ps = *p.panels;
+  check_panel_operands(p, vtype, colind, vals);
   DRX_REQUIRE(k >= 1 && y != nullptr, "bad output");
   DRX_REQUIRE(k == 1 || (ldx >= p.ncols && ldy >= p.nrows), "ldx/ldy too small");
   DRX_REQUIRE(static_cast<uint64_t>(p.ncols) * dtype_size(xtype) < (uint64_t(1) << 32),

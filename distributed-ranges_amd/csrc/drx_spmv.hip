@@ -28,27 +28,50 @@ Grid grid_for(int64_t nblocks) {
   Grid g;
   g.per_xcd = (nblocks + kXcds - 1) / kXcds;
   const int64_t want = g.per_xcd * kXcds;
-  const int64_t cap = int64_t(1) << 22;  // a multiple of 8; loop beyond it
+  // a multiple of 8 (workgroups keep their XCD as they loop past it)
+  const int64_t tuned = tuned_grid_cap();
+  const int64_t cap = tuned > 0 ? (tuned + kXcds - 1) / kXcds * kXcds : int64_t(1) << 22;
   g.grid = std::max<int64_t>(kXcds, std::min(want, cap));
   return g;
 }
 
 template <int MODE, bool BUFX, typename TV, typename TX, typename TY,
-          typename TA, typename TI>
-static void launch_fwd(const Plan &p, const FwdArgs &a, hipStream_t s) {
+          typename TA, typename TI, int VW>
+static void launch_fwd_vw(const Plan &p, const FwdArgs &a, hipStream_t s) {
   const dim3 grid(static_cast<unsigned>(a.grid));
   if (p.cap <= 1024)
-    hipLaunchKernelGGL((k_spmv_fwd<256, 1024, MODE, BUFX, TV, TX, TY, TA, TI>),
+    hipLaunchKernelGGL((k_spmv_fwd<256, 1024, MODE, BUFX, TV, TX, TY, TA, TI, VW>),
                        grid, dim3(256), 0, s, a);
   else if (p.cap <= 2048)
-    hipLaunchKernelGGL((k_spmv_fwd<256, 2048, MODE, BUFX, TV, TX, TY, TA, TI>),
+    hipLaunchKernelGGL((k_spmv_fwd<256, 2048, MODE, BUFX, TV, TX, TY, TA, TI, VW>),
                        grid, dim3(256), 0, s, a);
   else if (p.cap <= 4096)
-    hipLaunchKernelGGL((k_spmv_fwd<512, 4096, MODE, BUFX, TV, TX, TY, TA, TI>),
+    hipLaunchKernelGGL((k_spmv_fwd<512, 4096, MODE, BUFX, TV, TX, TY, TA, TI, VW>),
                        grid, dim3(512), 0, s, a);
   else
-    hipLaunchKernelGGL((k_spmv_fwd<1024, 8192, MODE, BUFX, TV, TX, TY, TA, TI>),
+    hipLaunchKernelGGL((k_spmv_fwd<1024, 8192, MODE, BUFX, TV, TX, TY, TA, TI, VW>),
                        grid, dim3(1024), 0, s, a);
+}
+
+// Pairs of entries per lane (VW = 2) for 4-byte-or-smaller values and
+// 32-bit indices when vals / colind are aligned to a pair. Knob "vec": 0
+// (default) pairs 16-bit values, and 4-byte values in column panels (where
+// it measured faster), 1 never pairs, 2 pairs wherever eligible.
+template <int MODE, bool BUFX, typename TV, typename TX, typename TY,
+          typename TA, typename TI>
+static void launch_fwd(const Plan &p, const FwdArgs &a, hipStream_t s) {
+  if constexpr (MODE != kPerm && sizeof(TI) == 4 && sizeof(TV) <= 4) {
+    const bool aligned =
+        reinterpret_cast<uintptr_t>(a.vals) % (2 * sizeof(TV)) == 0 &&
+        reinterpret_cast<uintptr_t>(a.colind) % 8 == 0;
+    const int vec = tuned_vec();
+    const bool want = vec == 2 || (vec == 0 && (sizeof(TV) == 2 || MODE == kRowMap));
+    if (aligned && want) {
+      launch_fwd_vw<MODE, BUFX, TV, TX, TY, TA, TI, 2>(p, a, s);
+      return;
+    }
+  }
+  launch_fwd_vw<MODE, BUFX, TV, TX, TY, TA, TI, 1>(p, a, s);
 }
 
 template <int MODE, typename TV, typename TX, typename TY, typename TA>
@@ -115,6 +138,13 @@ static void run_spmv_impl(const Plan &p, int vtype, const void *colind,
       static_cast<uint64_t>(std::max<int64_t>(p.ncols, 1)) * dtype_size(xtype);
   const bool bufx = xb < (uint64_t(1) << 32);
   a.xbytes = bufx ? static_cast<uint32_t>(xb) : 0;
+  a.ncols = p.ncols;
+  if constexpr (MODE != kRowMap) {
+    if (k >= 2 && tuned_mrhs()) {
+      run_mrhs(p, a, MODE, vtype, xtype, ytype, s);
+      return;
+    }
+  }
   if (vtype == DRX_F64 && xtype == DRX_F64 && ytype == DRX_F64)
     launch_fwd_t<MODE, double, double, double, double>(p, a, bufx, s);
   else if (vtype == DRX_F32 && xtype == DRX_F32 && ytype == DRX_F32)
@@ -136,11 +166,15 @@ void run_spmv(const Plan &p, int vtype, const void *colind, const void *vals,
               int xtype, const void *x, int64_t ldx, int ytype, void *y,
               int64_t ldy, int64_t k, double alpha, double beta,
               hipStream_t s) {
-  if (p.panels) {
+  // Column panels serve k = 1; with k >= 2 the row-major x chunks are KP
+  // times wider than a panel's L2-sized slice, and the product runs over
+  // the share's own arrays.
+  if (p.panels && (k == 1 || !tuned_mrhs())) {
     run_spmv_panels(p, vtype, colind, vals, xtype, x, ldx, ytype, y, ldy, k,
                     alpha, beta, s);
     return;
   }
+  if (p.panels) check_panel_operands(p, vtype, colind, vals);
   run_spmv_impl<kPlain>(p, vtype, colind, vals, nullptr, nullptr, xtype, x,
                         ldx, ytype, y, ldy, k, alpha, beta, s);
 }

@@ -29,6 +29,12 @@ c_u64 = ctypes.c_uint64
 SIGNATURES = {
     "drx_version": (c_int, []),
     "drx_last_error": (ctypes.c_char_p, []),
+    "drx_mm_read_csr": (c_int, [ctypes.c_char_p, c_int, c_int, c_int, c_int,
+                                ctypes.POINTER(c_i64), ctypes.POINTER(c_i64),
+                                ctypes.POINTER(c_i64), ctypes.POINTER(c_vp),
+                                ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)]),
+    "drx_host_free": (None, [c_vp]),
+    "drx_gather": (c_int, [c_vp, c_i64, c_int, c_vp, c_i64, c_vp, c_vp]),
     "drx_tune_set": (c_int, [ctypes.c_char_p, c_i64]),
     "drx_tune_get": (c_int, [ctypes.c_char_p, ctypes.POINTER(c_i64)]),
     "drx_plan_create": (c_int, [ctypes.POINTER(c_vp), c_int, c_vp, c_i64, c_i64,
@@ -126,3 +132,35 @@ def check(status):
         msg = lib().drx_last_error()
         raise DrxError(status, msg.decode() if msg else "")
     return status
+
+
+def tune_get(key):
+    """Current value of a process-wide tuning knob (drx_tune_get)."""
+    v = ctypes.c_int64()
+    check(lib().drx_tune_get(key.encode(), ctypes.byref(v)))
+    return v.value
+
+
+def tune_set(key, value):
+    """Set a process-wide tuning knob (drx_tune_set)."""
+    check(lib().drx_tune_set(key.encode(), int(value)))
+
+
+class tuned:
+    """Context manager setting tuning knobs and restoring them on exit:
+    ``with drx.tuned(mrhs=0): ...``."""
+
+    def __init__(self, **knobs):
+        self.knobs = knobs
+        self.saved = {}
+
+    def __enter__(self):
+        for k, v in self.knobs.items():
+            self.saved[k] = tune_get(k) if k != "block_nnz" else 0
+            tune_set(k, v)
+        return self
+
+    def __exit__(self, *exc):
+        for k, v in self.saved.items():
+            tune_set(k, v)
+        return False

@@ -79,3 +79,14 @@ def uniform_rows_csr(nrows, ncols, lmin, lmax, seed=0, vdtype=torch.float32,
                                           _p(colind), seed + 1, _s()))
     vals = normal(nnz, vdtype, seed=seed + 2, device=device)
     return CsrMatrix(rowptr, colind, vals, (nrows, ncols), block_nnz=block_nnz)
+
+
+def gather(idx, x, out=None, stream=None):
+    """out[i] = x[idx[i]] (drx_gather): int32 indices, 2/4/8-byte x."""
+    if idx.dtype != torch.int32:
+        raise ValueError("gather takes int32 indices")
+    if out is None:
+        out = torch.empty(idx.numel(), dtype=x.dtype, device=x.device)
+    check(lib().drx_gather(_p(idx), idx.numel(), DTYPE_CODE[x.dtype], _p(x), x.numel(),
+                           _p(out), _s(stream)))
+    return out

@@ -66,9 +66,21 @@ typedef struct drx_batch_s *drx_batch;
 /* ---- library ----------------------------------------------------------- */
 int drx_version(void);
 const char *drx_last_error(void);
-/* Process-wide tuning knobs. Key "block_nnz": default entries per row block
- * of new plans (0 restores the built-in default; env DRX_BLOCK_NNZ seeds it).
- * No counterpart in the reference. */
+/* Process-wide tuning knobs. No counterpart in the reference.
+ *   "block_nnz"   default entries per row block of new plans (0 restores the
+ *                 built-in default; env DRX_BLOCK_NNZ seeds it).
+ *   "mrhs"        k >= 2: 1 (default) copies x into row-major chunks on the
+ *                 call's stream (scratch from a per-device pool the library
+ *                 keeps) and gathers one contiguous vector per entry; 0
+ *                 gathers the k columns of x directly (no scratch).
+ *   "mrhs_gshift" lane-group shift of the k >= 2 kernel, 0..3 (default 1).
+ *   "vec"         entries per lane of the streaming loads for <= 4-byte
+ *                 values with 32-bit indices: 0 (default) two for 16-bit
+ *                 values and for column panels, one otherwise; 1 always
+ *                 one; 2 two wherever eligible (pairs need vals and colind
+ *                 aligned to a pair of entries; others use one).
+ *   "grid_cap"    most workgroups of a per-share launch (0 = one per row
+ *                 block); capped launches loop over the blocks. */
 int drx_tune_set(const char *key, int64_t value);
 int drx_tune_get(const char *key, int64_t *value);
 
@@ -267,9 +279,35 @@ int drx_gen_band(int64_t n, int64_t lo, int64_t hi, int itype, void *rowptr,
  * drx_transpose_arrays returns). */
 int drx_copy_device(void *dst, const void *src, int64_t bytes,
                     drx_stream_t stream);
+/* out[i] = x[idx[i]] for i < n (0 <= idx < nx, x below 4 GiB; elements of
+ * 2, 4 or 8 bytes by xtype). No counterpart in the reference: the
+ * benchmark times it on a product's own column indices to measure the
+ * random-gather rate that bounds products with scattered columns. */
+int drx_gather(const int32_t *idx, int64_t n, int xtype, const void *x,
+               int64_t nx, void *out, drx_stream_t stream);
 /* Convert between storage dtypes (round to nearest even). */
 int drx_convert(int src_dtype, const void *src, int dst_dtype, void *dst,
                 int64_t n, drx_stream_t stream);
+
+/* ---- Matrix Market input (host) ------------------------------------------
+ * Replaces dr::__detail::read_csr_matrix_base + convert_csr_base_to_csr and
+ * dr::read_csr (include/dr/detail/matrix_io.hpp:119-252), the reader behind
+ * dr::sp::mmread (include/dr/sp/util/matrix_io.hpp:68-84) and the mp
+ * examples' read_csr<double, long> (examples/mp/sparse_benchmark.cpp:58).
+ * Reads a coordinate file (field pattern -> value 1, any other field parsed
+ * as a number; symmetry general, or symmetric: an off-diagonal (i, j) is
+ * stored at (i, j) and (j, i)) with `nthreads` host threads (0 = all) into
+ * host CSR arrays that the library allocates and drx_host_free releases:
+ * rowptr (nrows + 1, 0-based), colind, vals (itype I32/I64; vtype F64/F32,
+ * values parsed directly in that type). Rows are sorted by column; entries
+ * with equal (row, column) are kept, in file order. Errors (status
+ * DRX_ERR_INVALID): unreadable file, not "matrix coordinate", unsupported
+ * symmetry, index out of bounds, more entry lines than the size line
+ * announces, malformed entry line. No device work. */
+int drx_mm_read_csr(const char *path, int one_indexed, int itype, int vtype,
+                    int nthreads, int64_t *nrows, int64_t *ncols, int64_t *nnz,
+                    void **rowptr, void **colind, void **vals);
+void drx_host_free(void *p);
 
 #ifdef __cplusplus
 }

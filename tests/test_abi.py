@@ -58,6 +58,14 @@ def test_argument_validation_without_gpu():
     assert L.drx_tune_set(b"block_nnz", 2048) == 0
     assert L.drx_tune_get(b"block_nnz", ctypes.byref(v)) == 0 and v.value == 2048
     assert L.drx_tune_set(b"block_nnz", 0) == 0
+    assert drx.tune_get("mrhs") == 1 and drx.tune_get("mrhs_gshift") == 1
+    assert drx.tune_get("vec") == 0 and drx.tune_get("grid_cap") == 0
+    assert L.drx_tune_set(b"vec", 3) == drx._lib.DRX_ERR_INVALID
+    assert L.drx_tune_set(b"mrhs", 2) == drx._lib.DRX_ERR_INVALID
+    assert L.drx_tune_set(b"mrhs_gshift", 4) == drx._lib.DRX_ERR_INVALID
+    with drx.tuned(mrhs=0, mrhs_gshift=3):
+        assert drx.tune_get("mrhs") == 0 and drx.tune_get("mrhs_gshift") == 3
+    assert drx.tune_get("mrhs") == 1 and drx.tune_get("mrhs_gshift") == 1
     assert L.drx_gen_laplacian_nnz(2, 1000) == 4996000
     assert L.drx_gen_laplacian_nnz(3, 256) == 117047296
     assert L.drx_gen_band_nnz(4000000, 16, 16) == 131999728

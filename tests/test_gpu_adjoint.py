@@ -41,7 +41,7 @@ def test_transpose_structure_bitexact(orc, itype):
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-@pytest.mark.parametrize("k", [1, 4])
+@pytest.mark.parametrize("k", [1, 4, 11])
 def test_adjoint_matches_oracle(orc, dtype, k):
     A = rand_csr(5000, 4000, 0.002, k)
     v = A.data.astype(np.float64 if dtype == torch.float64 else np.float32)
@@ -178,6 +178,12 @@ def test_adjoint_with_stored_transposed_values(orc, dtype):
     stored = Ad.rmatvec(dy)
     torch.cuda.synchronize()
     assert torch.equal(stored, via_perm)
+    # several right-hand sides: both value sources, bit for bit
+    dyk = torch.as_tensor(np.random.default_rng(3).uniform(-1, 1, (9, A.shape[0])).astype(v.dtype)).to(DEV)
+    stored_k = Ad.rmatvec(dyk)
+    via_perm_k = Ad.rmatvec(dyk, vals=Ad.vals.clone())
+    torch.cuda.synchronize()
+    assert torch.equal(stored_k, via_perm_k)
     Ad.vals.mul_(2)  # in-place update: the stored copy is stale now
     assert not Ad.stored_transposed_values(Ad.vals)
     dx2 = Ad.rmatvec(dy)

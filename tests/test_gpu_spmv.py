@@ -149,6 +149,89 @@ def test_long_and_empty_rows(orc, lens, dtype):
         assert componentwise_err(y, ystar, absy) < TOL[dtype]
 
 
+def _ragged_with_long_rows(m, n, seed, long_rows=(7, 700), long_len=9000):
+    """Rows of 0..60 entries (some empty) plus rows longer than the largest
+    LDS window (8192 entries), sorted distinct columns."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 61, m)
+    lens[rng.random(m) < 0.2] = 0
+    for r in long_rows:
+        lens[r] = long_len
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    ci = np.concatenate([np.sort(rng.choice(n, L, replace=False)) for L in lens]).astype(np.int64)
+    v = rng.uniform(-1, 1, ci.size)
+    return rp, ci, v
+
+
+@pytest.mark.parametrize("mrhs", [1, 0])
+@pytest.mark.parametrize("k", [2, 4, 5, 8, 9, 16, 20, 33])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_multi_rhs_paths(orc, mrhs, k, dtype):
+    """k right-hand sides through row-major chunks of x (mrhs=1: chunk widths
+    4, 8, 16 and several chunks for k > 16) and through direct column
+    gathers (mrhs=0), with empty rows and rows longer than every window;
+    alpha/beta and padded leading dimensions."""
+    rp, ci, v = _ragged_with_long_rows(1500, 10000, k)
+    v = v.astype(np.float64 if dtype == torch.float64 else np.float32)
+    m, n = 1500, 10000
+    rng = np.random.default_rng(100 + k)
+    ldx, ldy = n + 3, m + 5
+    x = np.zeros((k, ldx), v.dtype)
+    x[:, :n] = rng.uniform(-1, 1, (k, n))
+    y0 = rng.uniform(-1, 1, (k, ldy)).astype(v.dtype)
+    ys, absy = orc.spmv(rp, ci, v, x[:, :n])
+    with drx.tuned(mrhs=mrhs):
+        for itype, bn in ((torch.int32, 64), (torch.int64, 1024), (torch.int32, 4096)):
+            Ad = upload(rp, ci, v, (m, n), itype=itype, block_nnz=bn)
+            xd = torch.as_tensor(x).to(DEV)
+            y = torch.as_tensor(y0).to(DEV)
+            code = drx.F64 if dtype == torch.float64 else drx.F32
+            drx.check(drx.lib().drx_spmv(Ad.plan.handle, code, drx.core._p(Ad.colind),
+                                         drx.core._p(Ad.vals), code, drx.core._p(xd), ldx,
+                                         code, drx.core._p(y), ldy, k, 1.5, -0.25,
+                                         drx.core._s()))
+            torch.cuda.synchronize()
+            got = y.double().cpu().numpy()
+            expect = 1.5 * ys - 0.25 * y0[:, :m].astype(np.float64)
+            den = 1.5 * absy + 0.25 * np.abs(y0[:, :m])
+            assert (np.abs(got[:, :m] - expect) <= TOL[dtype] * den + 1e-300).all(), (itype, bn)
+            assert np.array_equal(got[:, m:], y0[:, m:].astype(np.float64))  # rows past m
+            Ad.close()
+
+
+@pytest.mark.parametrize("vdt", [torch.bfloat16, torch.float16])
+@pytest.mark.parametrize("xdt_same", [True, False])
+def test_low_precision_multi_rhs(orc, vdt, xdt_same):
+    """16-bit storage with FP32 accumulation for k = 2, 6 and 17 right-hand
+    sides (row-major chunks of 16-bit x: 4-, 16- and 32-byte gathers)."""
+    A = drx.gen.laplacian(3, 24, vdtype=torch.float32)
+    vq = A.vals.to(vdt)
+    rp, ci = A.rowptr.cpu().numpy(), A.colind.cpu().numpy()
+    vd = vq.double().cpu().numpy()
+    for k in (2, 6, 17):
+        x32 = drx.gen.uniform(k * A.shape[1], torch.float32, -1, 1, seed=k).reshape(k, -1)
+        xq = x32.to(vdt) if xdt_same else x32
+        y = A.matvec(xq, vals=vq)
+        torch.cuda.synchronize()
+        ystar, absy = orc.spmv(rp, ci, vd, xq.double().cpu().numpy())
+        assert componentwise_err(y.cpu().numpy(), ystar, absy) < 1e-5
+
+
+def test_multi_rhs_on_panel_plan(orc):
+    """A plan with column panels serves k >= 2 from the share's own arrays
+    (same operand contract: the pointers the layout was built from)."""
+    A = np_rand_csr(3000, 5000, 0.004, 77)
+    Ad = upload(A.indptr, A.indices, A.data.astype(np.float32), A.shape)
+    Ad.set_panels(512)
+    x = np.random.default_rng(5).uniform(-1, 1, (7, A.shape[1])).astype(np.float32)
+    y = Ad.matvec(torch.as_tensor(x).to(DEV))
+    torch.cuda.synchronize()
+    ys, absy = orc.spmv(A.indptr, A.indices, Ad.vals.cpu().numpy(), x)
+    assert componentwise_err(y.cpu().numpy(), ys, absy) < 1e-5
+    with pytest.raises(drx.DrxError):
+        Ad.matvec(torch.as_tensor(x).to(DEV), vals=Ad.vals.clone())
+
+
 def test_alpha_beta_and_ldx(orc):
     A = np_rand_csr(500, 400, 0.02, 1)
     Ad = upload(A.indptr, A.indices, A.data, A.shape)
@@ -357,7 +440,7 @@ def _batch_mats(nmat, seed, rows=(500, 4000), lens=(3, 64)):
     return mats
 
 
-@pytest.mark.parametrize("k", [1, 4])
+@pytest.mark.parametrize("k", [1, 4, 17])
 def test_batch_matches_single_calls_and_oracle(orc, k):
     """Forward + adjoint items of uneven matrices in one persistent launch:
     bit-identical to the single calls at the same block size and lane-group
@@ -372,7 +455,8 @@ def test_batch_matches_single_calls_and_oracle(orc, k):
     dxs = [torch.zeros_like(d) for d in dys]
     b = drx.Batch(forward=[(A, x, y) for A, x, y in zip(mats, xs, ys)],
                   adjoint=[(A, d, dx) for A, d, dx in zip(mats, dys, dxs)], k=k)
-    b.set_tuning(1024, 0, 4)
+    # same lane-group thresholds as the single calls (gshift 1 for k >= 2)
+    b.set_tuning(1024, drx.tune_get("mrhs_gshift") if k > 1 else 0, 4)
     b.run()
     torch.cuda.synchronize()
     for A, x, y, d, dx in zip(mats, xs, ys, dys, dxs):
@@ -416,3 +500,94 @@ def test_batch_graph_replays_and_autotune(orc, tmp_path):
             ys_, absy = orc.spmv(rp, ci, v, x.cpu().numpy())
             assert componentwise_err(g.cpu().numpy(), ys_, absy) < 1e-5
     b.close()
+
+
+def test_mmread_then_gemv(orc, tmp_path):
+    """A Matrix Market file read by the library's reader and multiplied on
+    the GPU equals the oracle on the scipy-read matrix."""
+    import scipy.io
+    rng = np.random.default_rng(31)
+    A = sp.random(2000, 1500, density=0.01, random_state=rng, format="coo")
+    A.data = rng.standard_normal(A.nnz)
+    path = str(tmp_path / "m.mtx")
+    scipy.io.mmwrite(path, A, field="real", symmetry="general")
+    Ad = drx.mmread(path, dtype=torch.float64)
+    ref = scipy.io.mmread(path).tocsr()
+    ref.sort_indices()
+    x = rng.uniform(-1, 1, 1500)
+    y = run(Ad, x, torch.float64)
+    ys, absy = orc.spmv(ref.indptr, ref.indices, ref.data, x)
+    assert componentwise_err(y, ys, absy) < 1e-12
+
+
+def _odd_window_rows(nb):
+    """Row lengths that give, at block_nnz nb (window 2 nb), a block whose
+    window is exactly full and starts at an odd entry ([nb + 1, 3 nb + 1):
+    rows of nb - 2 and nb + 2 entries starting in block 1, the next row
+    starting in block 3), next to long, empty and odd-length rows."""
+    return [nb + 1, nb - 2, nb + 2, 0, 3, 1, 300, 7, 0, 129, 5, 64, 63, 1, 2, 9000, 11]
+
+
+@pytest.mark.parametrize("vdt", [torch.float32, torch.bfloat16, torch.float16])
+@pytest.mark.parametrize("block_nnz", [64, 512, 1024])
+def test_paired_entry_loads_bitexact(orc, vdt, block_nnz):
+    """vec = 2 (two entries per lane per load) gives the same bits as one
+    entry per lane: full windows starting at odd entries, unaligned vals
+    (one-element offset: falls back to single entries), column panels."""
+    rng = np.random.default_rng(block_nnz)
+    lens = _odd_window_rows(block_nnz) + list(rng.integers(0, 40, 3000))
+    n = 20000
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    ci = np.concatenate([np.sort(rng.choice(n, L, replace=False)) for L in lens]).astype(np.int64)
+    v = rng.uniform(-1, 1, ci.size)
+    Ad = upload(rp, ci, v.astype(np.float32), (len(lens), n), block_nnz=block_nnz)
+    vq = Ad.vals.to(vdt)
+    x = drx.gen.uniform(n, torch.float32, -1, 1, seed=3)
+    xq = x.to(vdt) if vdt != torch.float32 else x
+    with drx.tuned(vec=1):
+        y1 = Ad.matvec(xq, vals=vq)
+    with drx.tuned(vec=2):
+        y2 = Ad.matvec(xq, vals=vq)
+        # vals starting one element into a larger buffer: not pair-aligned
+        buf = torch.zeros(vq.numel() + 1, dtype=vdt, device=DEV)
+        buf[1:] = vq
+        y3 = Ad.matvec(xq, vals=buf[1:])
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2) and torch.equal(y1, y3)
+    ys, absy = orc.spmv(rp, ci, vq.double().cpu().numpy(), xq.double().cpu().numpy())
+    assert componentwise_err(y2.cpu().numpy(), ys, absy) < 1e-5
+    if vdt == torch.float32:
+        Ad.set_panels(4096)
+        with drx.tuned(vec=1):
+            p1 = Ad.matvec(x)
+        with drx.tuned(vec=2):
+            p2 = Ad.matvec(x)
+        torch.cuda.synchronize()
+        assert torch.equal(p1, p2)
+        assert componentwise_err(p2.cpu().numpy(), ys, absy) < 1e-5
+
+
+@pytest.mark.parametrize("cap", [8, 64, 1000])
+def test_grid_cap_loops_over_blocks(orc, cap):
+    """A capped grid (workgroups loop over the row blocks, keeping their XCD)
+    gives the same bits as one workgroup per block."""
+    A = np_rand_csr(20000, 15000, 0.002, cap)
+    Ad = upload(A.indptr, A.indices, A.data.astype(np.float32), A.shape, block_nnz=256)
+    x = drx.gen.uniform(15000, torch.float32, -1, 1, seed=4)
+    y0 = Ad.matvec(x)
+    with drx.tuned(grid_cap=cap):
+        y1 = Ad.matvec(x)
+    torch.cuda.synchronize()
+    assert torch.equal(y0, y1)
+
+
+def test_gather_probe():
+    """drx_gather (the benchmark's gather-rate probe) equals torch indexing."""
+    x = torch.randn(70001, device=DEV)
+    idx = torch.randint(0, 70001, (123457,), dtype=torch.int32, device=DEV)
+    out = drx.gen.gather(idx, x)
+    assert torch.equal(out, x[idx.long()])
+    xd = x.double()
+    assert torch.equal(drx.gen.gather(idx, xd), xd[idx.long()])
+    xh = x.to(torch.bfloat16)
+    assert torch.equal(drx.gen.gather(idx, xh), xh[idx.long()])
