@@ -11,11 +11,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 BIN = os.path.join(HERE, "cpp", "_bin")
 
 
-def _run(name, timeout=300):
+def _run(name, timeout=300, args=()):
     exe = os.path.join(BIN, name)
     if not os.path.exists(exe):
         pytest.fail("%s not built (run __graft_entry__.build())" % exe)
-    p = subprocess.run([exe], capture_output=True, text=True, timeout=timeout)
+    p = subprocess.run([exe, *args], capture_output=True, text=True, timeout=timeout)
     print(p.stdout[-4000:])
     print(p.stderr[-2000:])
     return p
@@ -36,6 +36,33 @@ def test_host_generators_match_reference_fixtures():
                                      m["fnv1a64_vals"]), name
         seen += 1
     assert seen == 3 + 10 + 2
+
+
+def test_read_csr_matches_restated_reader(tmp_path):
+    """dr::read_csr<T, I> of the host interface (the library's reader) gives
+    the restated reference reader's arrays, for the type pairs the
+    reference's callers use."""
+    import sys
+    sys.path.insert(0, os.path.join(HERE, "..", "oracle"))
+    import mm_oracle
+    path = tmp_path / "s.mtx"
+    path.write_text("%%MatrixMarket matrix coordinate real symmetric\n% c\n"
+                    "5 5 7\n1 1 4\n2 1 -1\n3 2 -1.5\n5 5 0.125\n4 3 2e-3\n"
+                    "5 1 7\n3 3 1\n")
+    rp, ci, v, shape = mm_oracle.read_csr(str(path))
+    for kind in ("double_long", "float_int", "double_size_t"):
+        p = _run("test_mmio", timeout=60, args=(str(path), kind))
+        assert p.returncode == 0, p.stdout
+        lines = dict(line.split(" ", 1) for line in p.stdout.strip().splitlines())
+        assert lines["shape"] == "5 5 nnz %d" % len(ci)
+        assert [int(t) for t in lines["rowptr"].split()] == rp
+        assert [int(t) for t in lines["colind"].split()] == ci
+        cast = (lambda x: float(__import__("numpy").float32(x))) if kind == "float_int" else float
+        assert [float(t) for t in lines["vals"].split()] == [cast(x) for x in v]
+    bad = tmp_path / "b.mtx"
+    bad.write_text("%%MatrixMarket matrix coordinate real general\n2 2 1\n3 3 1\n")
+    p = _run("test_mmio", timeout=60, args=(str(bad), "double_long"))
+    assert p.returncode == 1 and "out of bounds" in p.stdout
 
 
 @pytest.mark.gpu
