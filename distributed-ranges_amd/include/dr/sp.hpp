@@ -5,8 +5,9 @@
 //   distributed_vector, duplicated_vector      .../distributed_vector.hpp,
 //                                              .../containers/duplicated_vector.hpp:12-46
 //   sparse_matrix (tiles in HBM, tile())       .../containers/sparse_matrix.hpp:135-425
-//   create_distributed                          .../util/matrix_io.hpp:20-66
+//   create_distributed, mmread                  .../util/matrix_io.hpp:20-84
 //   gemv(c, a, b[, scratch])                    .../algorithms/matrix/gemv.hpp:78-124
+//   flat_gemv, gemv_square, gemv_square_copy    .../algorithms/matrix/gemv.hpp:19-76,126-207
 //   __detail::local_gemv (per tile, c += A b)   .../algorithms/matrix/local_gemv.hpp:83-98
 //
 // A "device" of this model is a logical device: an entry of the list given
@@ -26,6 +27,7 @@
 #include <iterator>
 #include <limits>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "dr/detail/drx.hpp"
@@ -414,6 +416,22 @@ sparse_matrix<T, I> create_distributed(dr::views::csr_matrix_view<T, I> m) {
   return create_distributed(m, row_cyclic());
 }
 
+// Read a Matrix Market file and distribute it
+// (include/dr/sp/util/matrix_io.hpp:68-84).
+template <typename T, typename I = std::size_t>
+sparse_matrix<T, I> mmread(std::string file_path, const matrix_partition &partition,
+                           bool one_indexed = true) {
+  auto local = dr::read_csr<T, I>(file_path, one_indexed);
+  auto a = create_distributed(local, partition);
+  dr::__detail::destroy_csr_matrix_view(local, std::allocator<T>{});
+  return a;
+}
+
+template <typename T, typename I = std::size_t>
+sparse_matrix<T, I> mmread(std::string file_path, bool one_indexed = true) {
+  return mmread<T, I>(file_path, row_cyclic(), one_indexed);
+}
+
 // ---------------------------------------------------------------------------
 // gemv
 // ---------------------------------------------------------------------------
@@ -469,6 +487,94 @@ template <typename T, typename I>
 void gemv(distributed_vector<T> &c, sparse_matrix<T, I> &a, distributed_vector<T> &b) {
   duplicated_vector<T> b_duplicated(b.size());
   gemv(c, a, b, b_duplicated);
+}
+
+// flat_gemv (gemv.hpp:19-76): the same product with row tiles ({P, 1} grid),
+// b copied onto every device for this call. The reference walks each tile's
+// entries with one atomic add per entry; here each tile runs the CSR kernel
+// (accumulating, as gemv does).
+template <typename T, typename I>
+void flat_gemv(distributed_vector<T> &c, sparse_matrix<T, I> &a, distributed_vector<T> &b) {
+  assert(a.grid_shape()[1] == 1);
+  duplicated_vector<T> local_b(b.size());
+  gemv(c, a, b, local_b);
+}
+
+namespace __detail {
+// The tiles of a 2-D grid ({g0, g1}): row i takes its tiles in the rotation
+// k = (k_ + i) % g1 (gemv.hpp:133-136), each on its device's stream. The
+// reference's per-tile kernels add into c's segment with atomics, so the
+// tiles of a row may overlap; here a row's tiles are chained with events,
+// so one accumulating launch at a time writes a segment.
+template <typename T, typename I, typename BPtr>
+void gemv_tiles_2d(distributed_vector<T> &c, sparse_matrix<T, I> &a, BPtr &&b_of,
+                   bool copy_segments) {
+  const auto g = a.grid_shape();
+  auto csegs = c.segments();
+  std::vector<hipEvent_t> events;
+  for (std::size_t i = 0; i < g[0]; i++) {
+    hipEvent_t prev = nullptr;
+    for (std::size_t k_ = 0; k_ < g[1]; k_++) {
+      const std::size_t k = (k_ + i) % g[1];
+      auto t = a.tile(dr::index<I>(i, k));
+      hipStream_t q = queue(t.rank());
+      if (prev) dr::drx::hip_check(hipStreamWaitEvent(q, prev, 0), "hipStreamWaitEvent");
+      const T *bk = b_of(i, k, q, copy_segments);
+      assert(csegs[i].size == t.shape()[0]);
+      local_gemv(q, t, bk, csegs[i].data);
+      hipEvent_t e = nullptr;
+      dr::drx::hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+      dr::drx::hip_check(hipEventRecord(e, q), "hipEventRecord");
+      events.push_back(e);
+      prev = e;
+    }
+  }
+  wait_all();
+  for (auto e : events) (void)hipEventDestroy(e);
+}
+}  // namespace __detail
+
+// gemv_square (gemv.hpp:126-156): c += A b over a 2-D tile grid; b's
+// segment k is the right-hand side of tile column k (segments match the
+// tile width).
+template <typename T, typename I>
+void gemv_square(distributed_vector<T> &c, sparse_matrix<T, I> &a, distributed_vector<T> &b) {
+  assert(a.shape()[0] == c.size());
+  assert(a.shape()[1] == b.size());
+  assert(a.grid_shape()[0] == c.segments().size());
+  assert(a.grid_shape()[1] == b.segments().size());
+  auto bsegs = b.segments();
+  __detail::gemv_tiles_2d(
+      c, a, [&](std::size_t, std::size_t k, hipStream_t, bool) -> const T * {
+        return bsegs[k].data;
+      }, false);
+}
+
+// gemv_square_copy (gemv.hpp:158-207): as gemv_square, with each tile row's
+// b segments first copied into a full-length b on that row's device.
+template <typename T, typename I>
+void gemv_square_copy(distributed_vector<T> &c, sparse_matrix<T, I> &a,
+                      distributed_vector<T> &b) {
+  assert(a.shape()[0] == c.size());
+  assert(a.shape()[1] == b.size());
+  assert(a.grid_shape()[0] == c.segments().size());
+  assert(a.grid_shape()[1] == b.segments().size());
+  auto bsegs = b.segments();
+  const std::size_t tw = a.tile_shape()[1];
+  std::vector<device_vector<T>> local_b;
+  local_b.reserve(a.grid_shape()[0]);
+  for (std::size_t i = 0; i < a.grid_shape()[0]; i++)
+    local_b.emplace_back(b.size(), a.tile(dr::index<I>(i, 0)).rank());
+  __detail::gemv_tiles_2d(
+      c, a,
+      [&](std::size_t i, std::size_t k, hipStream_t q, bool) -> const T * {
+        T *dst = local_b[i].data() + k * tw;
+        dr::drx::hip_check(hipMemcpyAsync(dst, bsegs[k].data, bsegs[k].size * sizeof(T),
+                                          hipMemcpyDeviceToDevice, q),
+                           "copy b segment");
+        return dst;
+      },
+      true);
 }
 
 }  // namespace dr::sp

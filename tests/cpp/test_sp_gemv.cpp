@@ -101,4 +101,36 @@ TEST(SparseMatrix, GemvAccumulates) {
   sp::finalize();
 }
 
+// The 2-D tile grid forms (gemv.hpp:126-207: gemv_square, gemv_square_copy)
+// and flat_gemv (gemv.hpp:19-76), as examples/sp/gemv_benchmark.cpp calls
+// them: row sums of a random matrix, accumulated twice.
+template <typename F> void square_case(long m, long k, F &&fn, bool square_grid) {
+  for (std::size_t P = 1; P <= 4; P++) {
+    sp::init(P);
+    const std::size_t g1 = square_grid ? sp::nprocs() : 1;
+    sp::sparse_matrix<float> a({m, k}, 0.05f,
+                               sp::block_cyclic({sp::tile::div, sp::tile::div}, {sp::nprocs(), g1}));
+    sp::distributed_vector<float> b(k, 1.f);
+    sp::distributed_vector<float> c(m, 0.f);
+    fn(c, a, b);
+    fn(c, a, b);
+    std::vector<float> c_local(m);
+    sp::copy(c.begin(), c.end(), c_local.begin());
+    std::vector<float> c_ref(m, 0.f);
+    for (auto &&[index, v] : a) c_ref[index[0]] += 2 * v;
+    EXPECT_TRUE(harness::fp_equal(c_ref, c_local));
+    sp::finalize();
+  }
+}
+
+TEST(SparseMatrix, GemvSquare) {
+  square_case(400, 400, [](auto &c, auto &a, auto &b) { sp::gemv_square(c, a, b); }, true);
+}
+TEST(SparseMatrix, GemvSquareCopy) {
+  square_case(400, 400, [](auto &c, auto &a, auto &b) { sp::gemv_square_copy(c, a, b); }, true);
+}
+TEST(SparseMatrix, FlatGemv) {
+  square_case(300, 500, [](auto &c, auto &a, auto &b) { sp::flat_gemv(c, a, b); }, false);
+}
+
 int main(int argc, char **argv) { return harness::run_all(argc > 1 ? argv[1] : nullptr); }
