@@ -11,8 +11,10 @@ Also reported in the same JSON line (`forms`): the 2-D Laplacian FP64 form
 (1e6 rows, four rotating copies so the 256 MB MALL cannot hold them), the
 banded forward / adjoint / dvals / fused-residual forms (4M rows, band +-16),
 the low-precision storage forms (3-D Laplacian 256^3 and the Zipf matrix in
-BF16 / FP16 vs FP32 / FP64) with their error against the FP64 form, and the
-batched form.
+BF16 / FP16 vs FP32 / FP64) with their error against the FP64 form, the
+batched form, and the reference CPU benchmark's own input (GemvEq_DR: band
++-1000, 50000 rows, 8 right-hand sides, FP64) on the GPU and through the
+restated CPU path.
 
 cpu_baseline: the reference's CPU path (mp-bench GemvEq_DR without --sycl:
 csr_eq shares, one single-threaded rank per core, sequential local loop,
@@ -320,6 +322,57 @@ def form_batched(steps, nmat=512, ks=(1, 4, 16)):
     return out
 
 
+def form_gemveq_band(steps, cpu_seconds=10.0, with_cpu=True):
+    """The reference's own CPU benchmark input (benchmarks/gbench/mp/gemv.cpp:
+    GemvEq_DR): generate_band_csr<double, long>(50000, 1000, 1000), all-ones
+    values, width-8 slim matrix b[i + j*n] = i*j + 1, FP64 / int64. The GPU
+    form (k = 8 right-hand sides) and the restated GemvEq_DR CPU path time
+    the same matrix and right-hand sides."""
+    n, band, k = 50_000, 1000, 8
+    A = drx.gen.band(n, band, band, vdtype=torch.float64, itype=torch.int64, seed=1)
+    A.vals.fill_(1.0)
+    i = torch.arange(n, dtype=torch.float64, device="cuda")
+    x = torch.stack([i * j + 1 for j in range(k)])  # (k, n): column j of the slim matrix
+    y = torch.empty((k, n), dtype=torch.float64, device="cuda")
+    B = spmv_bytes(n, n, A.nnz, 8, 8, 8, 8, k=k)
+    _, ms = time_launches(lambda: A.matvec(x, y=y), steps, 3)
+    with drx.tuned(mrhs=0):
+        _, ms_cols = time_launches(lambda: A.matvec(x, y=y), steps, 3)
+    y = A.matvec(x)
+    torch.cuda.synchronize()
+    out = {"rows": n, "nnz": A.nnz, "k": k, "dtype": "f64/i64",
+           "GBps": round(B / ms / 1e6, 1), "frac": round(B / ms / 1e6 / HBM_PEAK_GBS, 4),
+           "us_per_call": round(ms * 1e3, 1),
+           "GBps_column_gathers": round(B / ms_cols / 1e6, 1)}
+    # exact answer: row r sums columns [r-1000, r+1000] of b
+    lo = torch.clamp(i - band, min=0)
+    hi = torch.clamp(i + band, max=n - 1)
+    cnt = hi - lo + 1
+    ssum = (lo + hi) * cnt / 2
+    exact = torch.stack([j * ssum + cnt for j in range(k)])
+    out["max_rel_err_vs_exact"] = float(((y - exact).abs() / exact.abs()).max())
+    if with_cpu:
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import _oracle  # the CPU restatement (test infrastructure)
+        orc = _oracle.Oracle()
+        rp, ci, v = A.rowptr.cpu().numpy(), A.colind.cpu().numpy(), A.vals.cpu().numpy()
+        xs = x.cpu().numpy()
+        try:
+            cores = len(os.sched_getaffinity(0))
+        except AttributeError:
+            cores = os.cpu_count() or 1
+        P = max(1, min(16, cores))
+        _, t1, _ = orc.gemveq_cpu_bench(rp, ci, v, xs, (n, n), P, warmup=0, reps=1)
+        reps = int(max(3, min(30, cpu_seconds / max(t1, 1e-3))))
+        res, med, _ = orc.gemveq_cpu_bench(rp, ci, v, xs, (n, n), P, warmup=1, reps=reps)
+        out["cpu_gemveq"] = {"GBps": round(B / med / 1e9, 3), "ms_per_call": round(med * 1e3, 2),
+                             "ranks": P, "reps": reps,
+                             "gpu_speedup": round(med * 1e3 / ms, 1),
+                             "max_abs_diff_gpu_vs_cpu": float(np.abs(res - y.cpu().numpy()).max())}
+    A.close()
+    return out
+
+
 # ---------------------------------------------------------------------------
 def read_traffic():
     """HBM bytes per launch of the headline kernel from the committed
@@ -391,7 +444,8 @@ def main():
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(A, x, y, args.cpu_seconds)
     forms = {}
-    want = set(["laplacian2d_f64", "band_adjoint", "lowprec", "batched"]) if args.forms == "all" else \
+    want = set(["laplacian2d_f64", "band_adjoint", "lowprec", "batched", "gemveq_band"]) \
+        if args.forms == "all" else \
         set() if args.forms == "none" else set(args.forms.split(","))
     steps_f = max(5, min(args.steps, 20))
     if "lowprec" in want:
@@ -408,6 +462,9 @@ def main():
     if "batched" in want:
         log("form batched ...")
         forms["batched"] = form_batched(steps_f)
+    if "gemveq_band" in want:
+        log("form gemveq_band ...")
+        forms["gemveq_band"] = form_gemveq_band(steps_f, args.cpu_seconds, not args.no_cpu)
     if forms:
         result["forms"] = forms
     print(json.dumps(result), flush=True)

@@ -467,9 +467,10 @@ double gemveq_cpu_bench(std::size_t m, std::size_t n, std::size_t nnz,
       }
     }
   };
+  // Every rank, rank 0 included, runs on a thread of its own, so pinning a
+  // rank never changes the caller's own CPU affinity.
   std::vector<std::thread> th;
-  for (std::size_t r = 1; r < P; r++) th.emplace_back(rank_main, r);
-  rank_main(0);
+  for (std::size_t r = 0; r < P; r++) th.emplace_back(rank_main, r);
   for (auto &t : th) t.join();
   for (std::size_t i = 0; i < t_rank0.size(); i++) times[i] = t_rank0[i];
   std::vector<double> s = t_rank0;
