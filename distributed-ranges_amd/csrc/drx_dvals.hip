@@ -1,9 +1,9 @@
 // drx_dvals.hip -- gradient of y = A x with respect to the stored values
 // (dvals), and the fused forward + residual + dvals pass, for gfx950.
 //
-// Both walk the plan's row blocks like the forward kernel. dvals uses lane
-// groups per row sized by the block's mean row length, so each group's
-// entries are contiguous (coalesced colind reads, dvals writes). The fused
+// Both walk the plan's row blocks like the forward kernel. dvals streams the
+// block's column indices, looks each entry's row up in LDS and writes dvals
+// with coalesced stores (dy of the block's rows staged in LDS). The fused
 // kernel stages both the products and the gathered x values of the window in
 // LDS, so that after the row sums (and the residual r = y - t) the dvals of
 // the row are written from LDS without reading colind or x a second time.
@@ -35,48 +35,121 @@ struct DvalsArgs {
   double alpha, beta;
 };
 
-template <int G, typename TX, typename TD, typename TA, typename TI>
-__device__ __forceinline__ void dvals_rows(const DvalsArgs &a, int64_t ra,
-                                           int64_t rb) {
+// Row index (within the block) of every window entry, written by lane
+// groups of G lanes per row; row offsets from LDS or, for blocks with more
+// rows than the LDS holds, from rowptr.
+template <int WG, int G, typename TI>
+__device__ __forceinline__ void fill_rowid(const DvalsArgs &a, const int *rp_s,
+                                           bool rows_lds, int64_t r0, int64_t e0,
+                                           int64_t nr, int *rowid) {
+  const TI *rowptr = static_cast<const TI *>(a.rowptr);
+  const int lane = threadIdx.x & (G - 1);
+  for (int64_t i = threadIdx.x / G; i < nr; i += WG / G) {
+    int s, e;
+    if (rows_lds) {
+      s = rp_s[i];
+      e = rp_s[i + 1];
+    } else {
+      s = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r0 + i) - e0);
+      e = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r0 + i + 1) - e0);
+    }
+    for (int q = s + lane; q < e; q += G) rowid[q] = static_cast<int>(i);
+  }
+}
+
+// One row block per workgroup, like the forward kernel: the window's column
+// indices are streamed (coalesced, non-temporal) and x gathered into
+// registers, each entry's row is looked up in LDS (rowid), dy of the
+// block's rows is staged in LDS, and dvals is written with coalesced stores.
+// A last row longer than the window is done by the whole workgroup.
+template <int WG, int CAP, typename TX, typename TD, typename TA, typename TI>
+__global__ __launch_bounds__(WG) void k_dvals(DvalsArgs a) {
+  constexpr int T = CAP / WG;
+  constexpr int ROWCAP = CAP / 2;
+  __shared__ int rp_s[ROWCAP + 1];
+  __shared__ int rowid[CAP];
+  __shared__ TA dy_s[ROWCAP];
+  const int tid = threadIdx.x;
   const TI *rowptr = static_cast<const TI *>(a.rowptr);
   const TI *colind = static_cast<const TI *>(a.colind);
   const TX *x = static_cast<const TX *>(a.x);
   const TX *dy = static_cast<const TX *>(a.dy);
   TD *dv = static_cast<TD *>(a.dvals);
   const TA alpha = static_cast<TA>(a.alpha), beta = static_cast<TA>(a.beta);
-  const int lane = threadIdx.x & (G - 1);
-  constexpr int NG = kWG / G;
-  for (int64_t r = ra + threadIdx.x / G; r < rb; r += NG) {
-    const int64_t s = row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r);
-    const int64_t e = row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r + 1);
-    for (int64_t q = s + lane; q < e; q += G) {
-      const int64_t col = static_cast<int64_t>(ld_stream(colind + q));
-      TA acc = 0;
-      for (int64_t j = 0; j < a.k; j++)
-        acc += to_acc<TA>(dy[r + j * a.lddy]) * to_acc<TA>(x[col + j * a.ldx]);
-      TA out = alpha * acc;
-      if (beta != TA(0)) out += beta * static_cast<TA>(dv[q]);
-      dv[q] = static_cast<TD>(out);
-    }
-  }
-}
-
-template <typename TX, typename TD, typename TA, typename TI>
-__global__ __launch_bounds__(kWG) void k_dvals(DvalsArgs a) {
   for (int64_t wi = blockIdx.x; wi < a.per_xcd * kXcds; wi += a.grid) {
     const int64_t blk = map_block(wi, a.per_xcd);
     if (blk >= a.nblocks) continue;
-    const int64_t r0 = a.rec[blk].r0, r1 = a.rec[blk + 1].r0;
-    if (r0 >= r1) continue;
-    const int64_t avg = (a.rec[blk + 1].e0 - a.rec[blk].e0) / (r1 - r0);
-    if (avg <= 3)
-      dvals_rows<1, TX, TD, TA, TI>(a, r0, r1);
-    else if (avg <= 12)
-      dvals_rows<4, TX, TD, TA, TI>(a, r0, r1);
-    else if (avg <= 48)
-      dvals_rows<16, TX, TD, TA, TI>(a, r0, r1);
-    else
-      dvals_rows<64, TX, TD, TA, TI>(a, r0, r1);
+    const BlockRec b = a.rec[blk];
+    const int64_t r1 = a.rec[blk + 1].r0, e1 = a.rec[blk + 1].e0;
+    if (b.r0 >= r1) continue;
+    const int n_s = static_cast<int>(b.es - b.e0);
+    const int64_t nr = (b.es == e1 ? r1 : r1 - 1) - b.r0;
+    const bool rows_lds = nr < ROWCAP;
+    uint32_t c[T];
+    {
+      const rsrc_t rc = make_rsrc(colind + b.e0, n_s * sizeof(TI));
+#pragma unroll
+      for (int u = 0; u < T; u++)
+        c[u] = col32(buf_load<TI, kNT>(rc, (tid + u * WG) * sizeof(TI)));
+    }
+    if (rows_lds) {
+      for (int i = tid; i <= nr; i += WG)
+        rp_s[i] = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, b.r0 + i) - b.e0);
+    }
+    __syncthreads();
+    if (nr > 0) {
+      const int64_t avg = n_s / nr;
+      if (avg <= 3)
+        fill_rowid<WG, 1, TI>(a, rp_s, rows_lds, b.r0, b.e0, nr, rowid);
+      else if (avg <= 12)
+        fill_rowid<WG, 4, TI>(a, rp_s, rows_lds, b.r0, b.e0, nr, rowid);
+      else if (avg <= 48)
+        fill_rowid<WG, 16, TI>(a, rp_s, rows_lds, b.r0, b.e0, nr, rowid);
+      else
+        fill_rowid<WG, 64, TI>(a, rp_s, rows_lds, b.r0, b.e0, nr, rowid);
+    }
+    TA acc[T];
+#pragma unroll
+    for (int u = 0; u < T; u++) acc[u] = 0;
+    for (int64_t j = 0; j < a.k; j++) {
+      const TX *xj = x + j * a.ldx;
+      const TX *dyj = dy + j * a.lddy + b.r0;
+      if (rows_lds) {
+        for (int i = tid; i < nr; i += WG) dy_s[i] = to_acc<TA>(dyj[i]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < T; u++) {
+        const int idx = tid + u * WG;
+        if (idx < n_s) {
+          const int ri = rowid[idx];
+          const TA d = rows_lds ? dy_s[ri] : to_acc<TA>(dyj[ri]);
+          acc[u] += d * to_acc<TA>(xj[c[u]]);
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < T; u++) {
+      const int idx = tid + u * WG;
+      if (idx < n_s) {
+        TA out = alpha * acc[u];
+        if (beta != TA(0)) out += beta * static_cast<TA>(dv[b.e0 + idx]);
+        dv[b.e0 + idx] = static_cast<TD>(out);
+      }
+    }
+    if (b.es < e1) {
+      const int64_t last = r1 - 1;
+      for (int64_t q = b.es + tid; q < e1; q += WG) {
+        const int64_t col = static_cast<int64_t>(ld_stream(colind + q));
+        TA s = 0;
+        for (int64_t j = 0; j < a.k; j++)
+          s += to_acc<TA>(dy[last + j * a.lddy]) * to_acc<TA>(x[col + j * a.ldx]);
+        TA out = alpha * s;
+        if (beta != TA(0)) out += beta * static_cast<TA>(dv[q]);
+        dv[q] = static_cast<TD>(out);
+      }
+    }
   }
 }
 
@@ -215,14 +288,27 @@ void run_dvals(const Plan &p, const void *colind, int xtype, const void *x,
   a.dvals = dvals;
   a.alpha = alpha;
   a.beta = beta;
-  const dim3 grid(static_cast<unsigned>(a.grid)), block(kWG);
+  const dim3 grid(static_cast<unsigned>(a.grid));
   const bool i32 = p.itype == DRX_I32;
-#define DRX_DV(TX, TD, TA)                                                     \
+#define DRX_DV_I(WG, CAP, TX, TD, TA)                                          \
   do {                                                                         \
     if (i32)                                                                   \
-      hipLaunchKernelGGL((k_dvals<TX, TD, TA, int32_t>), grid, block, 0, s, a); \
+      hipLaunchKernelGGL((k_dvals<WG, CAP, TX, TD, TA, int32_t>), grid,        \
+                         dim3(WG), 0, s, a);                                   \
     else                                                                       \
-      hipLaunchKernelGGL((k_dvals<TX, TD, TA, int64_t>), grid, block, 0, s, a); \
+      hipLaunchKernelGGL((k_dvals<WG, CAP, TX, TD, TA, int64_t>), grid,        \
+                         dim3(WG), 0, s, a);                                   \
+  } while (0)
+#define DRX_DV(TX, TD, TA)                                                     \
+  do {                                                                         \
+    if (p.cap <= 1024)                                                         \
+      DRX_DV_I(256, 1024, TX, TD, TA);                                         \
+    else if (p.cap <= 2048)                                                    \
+      DRX_DV_I(256, 2048, TX, TD, TA);                                         \
+    else if (p.cap <= 4096)                                                    \
+      DRX_DV_I(512, 4096, TX, TD, TA);                                         \
+    else                                                                       \
+      DRX_DV_I(1024, 8192, TX, TD, TA);                                        \
   } while (0)
   if (xtype == DRX_F64 && dvtype == DRX_F64)
     DRX_DV(double, double, double);
@@ -235,6 +321,7 @@ void run_dvals(const Plan &p, const void *colind, int xtype, const void *x,
   else
     throw Error(DRX_ERR_UNSUPPORTED, "unsupported (xtype, dvtype)");
 #undef DRX_DV
+#undef DRX_DV_I
   DRX_HIP(hipGetLastError());
 }
 

@@ -95,6 +95,30 @@ def test_dvals_matches_oracle(orc, dtype, k):
     assert np.abs(d.cpu().numpy() - ref).max() <= tol * scale
 
 
+@pytest.mark.parametrize("block_nnz", [64, 1024, 4096])
+@pytest.mark.parametrize("k", [1, 2])
+def test_dvals_long_short_and_empty_rows(orc, block_nnz, k):
+    """Rows longer than the window (whole-workgroup path), runs of empty and
+    one-entry rows (more rows than the window's row table: row ids from
+    rowptr), alpha and beta."""
+    rng = np.random.default_rng(block_nnz + k)
+    lens = [0] * 300 + [1] * 500 + [9000, 0, 3, 20000] + list(rng.integers(0, 50, 2000)) + [0] * 100
+    n = 30000
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    ci = np.concatenate([np.sort(rng.choice(n, L, replace=False)) for L in lens]).astype(np.int64)
+    v = rng.uniform(-1, 1, ci.size)
+    Ad = drx.from_host(rp, ci, v, (len(lens), n), block_nnz=block_nnz)
+    x = rng.uniform(-1, 1, (k, n))
+    dy = rng.uniform(-1, 1, (k, len(lens)))
+    sq = (lambda a: a) if k > 1 else (lambda a: a[0])
+    d0 = rng.uniform(-1, 1, ci.size)
+    out = torch.as_tensor(d0).to(DEV)
+    Ad.dvals(torch.as_tensor(sq(x)).to(DEV), torch.as_tensor(sq(dy)).to(DEV), out=out,
+             alpha=2.0, beta=-0.5)
+    ref = 2.0 * orc.dvals(rp, ci, sq(x), sq(dy)) - 0.5 * d0
+    assert np.abs(out.cpu().numpy() - ref).max() <= 1e-14 * (2 * k + 1)
+
+
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_fused_residual_matches_separate(orc, dtype):
     A = drx.gen.band(100000, 16, 16, vdtype=dtype, seed=11)
