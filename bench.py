@@ -100,7 +100,23 @@ def headline(steps, warmup):
     layout = {"panels": npan, "panel_cols": width, "row_segments": nseg,
               "plain_csr_ms_per_step": round(plain_ms, 4),
               "plain_csr_GBps": round(B / plain_ms / 1e6, 2)}
-    return A, x, y, B, wall_ms, ev_ms, layout
+    # The bound that matters here: random 4-byte gathers inside one panel's x
+    # slice. drx_gather does only that (streamed indices, gathered values
+    # written out) for one panel's share of the entries.
+    per = A.nnz // npan
+    g = torch.Generator(device="cuda").manual_seed(3)
+    idx = torch.randint(0, width, (per,), dtype=torch.int32, device="cuda", generator=g)
+    out = torch.empty(per, dtype=torch.float32, device="cuda")
+    xs = x[:width].contiguous()
+    _, g_ms = time_launches(lambda: drx.gen.gather(idx, xs, out=out), max(5, steps // 2), 2)
+    del idx, out
+    torch.cuda.empty_cache()
+    gather = {"G_gathers_per_s": round(per / g_ms / 1e6, 1),
+              "ms_per_step": round(g_ms * npan, 4),
+              "kernel_frac_of_gather_ceiling": round(g_ms * npan / ev_ms, 4),
+              "probe": "drx_gather: %d random int32 indices into %d floats (one panel's x "
+                       "slice), x %d panels" % (per, width, npan)}
+    return A, x, y, B, wall_ms, ev_ms, layout, gather
 
 
 def cpu_baseline(A, x, y_gpu, target_s=10.0):
@@ -416,7 +432,7 @@ def main():
               % args.gpus, file=sys.stderr)
         sys.exit(2)
     torch.cuda.set_device(0)
-    A, x, y, B, wall_ms, ev_ms, layout = headline(args.steps, args.warmup)
+    A, x, y, B, wall_ms, ev_ms, layout, gather = headline(args.steps, args.warmup)
     value = B / (wall_ms * 1e-3) / 1e9
     achieved = B / (ev_ms * 1e-3) / 1e9
     result = {
@@ -438,7 +454,8 @@ def main():
                      "kernel": "drx::k_spmv_fwd<256,2048,kRowMap,f32,i32> (one launch per "
                                "column panel) + k_scale_y; achieved = bytes / summed time of "
                                "one step's launches",
-                     "ms_per_step_kernels": round(ev_ms, 4)},
+                     "ms_per_step_kernels": round(ev_ms, 4),
+                     "gather_ceiling": gather},
     }
     if not args.no_cpu:
         log("cpu baseline ...")

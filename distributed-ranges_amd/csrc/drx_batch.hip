@@ -77,13 +77,35 @@ struct Batch {
   int64_t njobs = 0, xr_threads = 0;
 };
 
+// A record every lane of the workgroup reads at the same address, copied
+// word by word through readfirstlane so that it lives in scalar registers.
+__device__ __forceinline__ FwdArgs load_uniform(const FwdArgs *p) {
+  static_assert(sizeof(FwdArgs) % 4 == 0, "word copy");
+  FwdArgs out;
+  const uint32_t *src = reinterpret_cast<const uint32_t *>(p);
+  uint32_t *dst = reinterpret_cast<uint32_t *>(&out);
+#pragma unroll
+  for (int i = 0; i < static_cast<int>(sizeof(FwdArgs) / 4); i++)
+    dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  return out;
+}
+
+// Waves per SIMD the batch kernel is compiled for: the most that needs no
+// register spills (5 for 4-byte accumulation in 256-lane workgroups, else
+// the compiler's 4).
+template <int WG, typename TA> constexpr int batch_waves() {
+  return (sizeof(TA) == 8 || WG >= 512) ? 4 : 5;
+}
+
 template <int WG, int CAP, typename TV, typename TX, typename TY, typename TA,
           typename TI>
-__global__ __launch_bounds__(WG) void k_batch(const FwdArgs *items,
+__global__ __launch_bounds__(WG)
+__attribute__((amdgpu_waves_per_eu(batch_waves<WG, TA>(), 8)))
+void k_batch(const FwdArgs *items,
                                               const int *modes,
                                               const int64_t *off,
                                               int64_t nitems, int64_t total) {
-  __shared__ FwdSmem<WG, CAP, false, TA> sm;
+  __shared__ FwdSmem<WG, CAP, false, TA, TI> sm;
   const int64_t per_xcd = (total + kXcds - 1) / kXcds;
   const int64_t xcd = blockIdx.x % kXcds;
   const int64_t lw = blockIdx.x / kXcds, nw = gridDim.x / kXcds;
@@ -102,12 +124,12 @@ __global__ __launch_bounds__(WG) void k_batch(const FwdArgs *items,
   // The item's argument record is copied once per item change: read from
   // global memory inside the block body it would be re-loaded after every
   // store to y.
-  FwdArgs cur = items[it];
+  FwdArgs cur = load_uniform(items + it);
   int mode = modes[it];
   for (; w < hi; w += nw) {
     if (off[it + 1] <= w) {
       while (off[it + 1] <= w) it++;
-      cur = items[it];
+      cur = load_uniform(items + it);
       mode = modes[it];
     }
     const int64_t blk = w - off[it];
@@ -147,12 +169,12 @@ __global__ __launch_bounds__(WG) void k_batch_mrhs(const FwdArgs *items,
     else
       itb = mid;
   }
-  FwdArgs cur = items[it];
+  FwdArgs cur = load_uniform(items + it);
   int mode = modes[it];
   for (; w < hi; w += nw) {
     if (off[it + 1] <= w) {
       while (off[it + 1] <= w) it++;
-      cur = items[it];
+      cur = load_uniform(items + it);
       mode = modes[it];
     }
     const int64_t blk = w - off[it];

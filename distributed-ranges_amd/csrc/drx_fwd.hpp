@@ -67,7 +67,7 @@ __device__ __forceinline__ TA block_sum(TA v, TA *red) {
 // LDS holds, read from rowptr.
 template <int WG, int G, int MODE, typename TA, typename TY, typename TI>
 __device__ __forceinline__ void reduce_rows(const TA *prod, const int *rp_s,
-                                            const int64_t *map_s, const TA *ypre,
+                                            const TI *map_s, const TA *ypre,
                                             bool rows_lds, const FwdArgs &a,
                                             int64_t r0, int64_t e0, int64_t nr,
                                             TY *yj, TA alpha, TA beta) {
@@ -90,7 +90,7 @@ __device__ __forceinline__ void reduce_rows(const TA *prod, const int *rp_s,
     if (lane == 0) {
       if constexpr (MODE == kRowMap) {
         if (ypre != nullptr && rows_lds) {
-          yj[map_s[i]] = from_acc<TY>(ypre[i] + alpha * acc);
+          yj[static_cast<int64_t>(map_s[i])] = from_acc<TY>(ypre[i] + alpha * acc);
         } else {
           const int64_t row = static_cast<int64_t>(rowmap[r0 + i]);
           yj[row] = from_acc<TY>(to_acc<TA>(yj[row]) + alpha * acc);
@@ -106,7 +106,7 @@ __device__ __forceinline__ void reduce_rows(const TA *prod, const int *rp_s,
 
 template <int WG, int MODE, typename TA, typename TY, typename TI>
 __device__ __forceinline__ void reduce_rows_any(int n_s, const TA *prod,
-                                                const int *rp_s, const int64_t *map_s,
+                                                const int *rp_s, const TI *map_s,
                                                 const TA *ypre, bool rows_lds,
                                                 const FwdArgs &a, int64_t r0,
                                                 int64_t e0, int64_t nr, TY *yj,
@@ -140,15 +140,16 @@ __device__ __forceinline__ TV long_val(const TV *vals, const TI *perm,
     return ld_stream(vals + q);
 }
 
-// LDS of one workgroup of the forward kernels.
-template <int WG, int CAP, bool ROWMAP, typename TA> struct FwdSmem {
+// LDS of one workgroup of the forward kernels (row-map entries have the
+// index type).
+template <int WG, int CAP, bool ROWMAP, typename TA, typename TI> struct FwdSmem {
   static constexpr int ROWCAP = CAP / 2;
   static constexpr int MAPCAP = ROWMAP ? ROWCAP : 1;
   TA prod[CAP];
-  int rp_s[ROWCAP + 1];
-  int64_t map_s[MAPCAP];  // kRowMap: output rows of the block
-  TA ypre[MAPCAP];        // kRowMap, k = 1: their current y
+  TA ypre[MAPCAP];        // kRowMap, k = 1: the current y of the block's rows
   TA red[WG / 64];
+  TI map_s[MAPCAP];       // kRowMap: output rows of the block
+  int rp_s[ROWCAP + 1];
 };
 
 // Storage value from its raw bits (2- or 4-byte types).
@@ -172,14 +173,14 @@ template <typename TV, typename U> __device__ __forceinline__ TV bits_to(U u) {
 template <int WG, int CAP, int MODE, bool BUFX, typename TV, typename TX,
           typename TY, typename TA, typename TI, int VW = 1>
 __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
-                                              FwdSmem<WG, CAP, MODE == kRowMap, TA> &sm) {
+                                              FwdSmem<WG, CAP, MODE == kRowMap, TA, TI> &sm) {
   static_assert(VW == 1 || (VW == 2 && MODE != kPerm && sizeof(TI) == 4 && sizeof(TV) <= 4),
                 "pairs of entries: plain or row-map mode, 32-bit indices, <= 4-byte values");
   constexpr int T = CAP / WG;
   constexpr int ROWCAP = CAP / 2;
   TA *prod = sm.prod;
   int *rp_s = sm.rp_s;
-  int64_t *map_s = sm.map_s;
+  TI *map_s = sm.map_s;
   TA *ypre = sm.ypre;
   TA *red = sm.red;
   const int tid = threadIdx.x;
@@ -267,7 +268,7 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
             row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, b.r0 + i) - b.e0);
         if constexpr (MODE == kRowMap) {
           if (i < nr) {
-            const int64_t row = static_cast<int64_t>(static_cast<const TI *>(a.rowmap)[b.r0 + i]);
+            const TI row = static_cast<const TI *>(a.rowmap)[b.r0 + i];
             map_s[i] = row;
             if (a.k == 1) ypre[i] = to_acc<TA>(y[row]);
           }
@@ -350,8 +351,8 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
 // FP32 raise the structured forms by 7-17 % over the compiler's default (4-5
 // waves); trading LDS for an eighth FP64 workgroup spilled registers and
 // lost 30 %.
-template <int WG, int CAP, int MODE, typename TA> constexpr int fwd_waves() {
-  constexpr int lds = static_cast<int>(sizeof(FwdSmem<WG, CAP, MODE == kRowMap, TA>));
+template <int WG, int CAP, int MODE, typename TA, typename TI> constexpr int fwd_waves() {
+  constexpr int lds = static_cast<int>(sizeof(FwdSmem<WG, CAP, MODE == kRowMap, TA, TI>));
   constexpr int wgs = (160 * 1024) / lds;
   constexpr int w = wgs * (WG / 256);
   return w > 8 ? 8 : (w < 1 ? 1 : w);
@@ -360,9 +361,9 @@ template <int WG, int CAP, int MODE, typename TA> constexpr int fwd_waves() {
 template <int WG, int CAP, int MODE, bool BUFX, typename TV, typename TX,
           typename TY, typename TA, typename TI, int VW = 1>
 __global__ __launch_bounds__(WG)
-__attribute__((amdgpu_waves_per_eu(fwd_waves<WG, CAP, MODE, TA>(), 8)))
+__attribute__((amdgpu_waves_per_eu(fwd_waves<WG, CAP, MODE, TA, TI>(), 8)))
 void k_spmv_fwd(FwdArgs a) {
-  __shared__ FwdSmem<WG, CAP, MODE == kRowMap, TA> sm;
+  __shared__ FwdSmem<WG, CAP, MODE == kRowMap, TA, TI> sm;
   for (int64_t wi = blockIdx.x; wi < a.per_xcd * kXcds; wi += a.grid) {
     const int64_t blk = map_block(wi, a.per_xcd);
     if (blk >= a.nblocks) continue;
