@@ -230,14 +230,18 @@ static void build_panels(Plan &p, PanelSet &ps, const TI *colind,
     key = key_sorted = nullptr;
     flag = segid = nullptr;
     d_pstart = d_pseg = nullptr;
-    // One plan per panel over its segments (absolute entry offsets, base =
-    // the panel's first entry, sentinel = next panel's first entry).
+    // One plan per panel over its segments (absolute entry offsets, sentinel
+    // = next panel's first entry). The plan's base is the panel's first
+    // entry rounded down to an even entry, so that the panel's arrays start
+    // pair-aligned (two entries per lane per load); a panel starting at an
+    // odd entry then has one leading entry that belongs to none of its rows.
     const TI *segp = static_cast<const TI *>(ps.seg_ptr);
     for (int64_t q = 0; q < ps.npanels; q++) {
       const int64_t rows = ps.pseg[q + 1] - ps.pseg[q];
-      const int64_t pn = ps.pstart[q + 1] - ps.pstart[q];
-      ps.plans.push_back(make_plan(p.itype, segp + ps.pseg[q], ps.pstart[q], rows,
-                                   p.ncols, pn, 1, p.block_nnz, s, false, false));
+      const int64_t base = ps.pstart[q] & ~int64_t(1);
+      const int64_t pn = ps.pstart[q + 1] - base;
+      ps.plans.push_back(make_plan(p.itype, segp + ps.pseg[q], base, rows, p.ncols, pn, 1,
+                                   p.block_nnz, s, false, false));
     }
     DRX_HIP(hipStreamSynchronize(s));
   } catch (...) {
@@ -293,9 +297,10 @@ void run_spmv_panels(const Plan &p, int vtype, const void *colind,
   const size_t isz = p.itype == DRX_I32 ? 4 : 8, vsz = dtype_size(vtype);
   for (int64_t q = 0; q < ps.npanels; q++) {
     if (ps.pstart[q + 1] == ps.pstart[q]) continue;
+    const int64_t base = ps.pstart[q] & ~int64_t(1);  // the panel plan's base
     run_spmv_rowmap(*ps.plans[q], vtype,
-                    static_cast<const char *>(ps.pcol) + ps.pstart[q] * isz,
-                    static_cast<const char *>(ps.pval) + ps.pstart[q] * vsz,
+                    static_cast<const char *>(ps.pcol) + base * isz,
+                    static_cast<const char *>(ps.pval) + base * vsz,
                     static_cast<const char *>(ps.seg_row) + ps.pseg[q] * isz, xtype,
                     x, ldx, ytype, y, ldy, k, alpha, s);
   }
