@@ -332,7 +332,7 @@ int drx_tune_set(const char *key, int64_t value) {
                   "block_nnz must be 0 or a power of two in [64, 4096]");
       g_block_nnz.store(value);
     } else if (std::strcmp(key, "mrhs") == 0) {
-      DRX_REQUIRE(value == 0 || value == 1, "mrhs must be 0 or 1");
+      DRX_REQUIRE(value >= 0 && value <= 2, "mrhs must be 0, 1 or 2");
       g_mrhs.store(static_cast<int>(value));
     } else if (std::strcmp(key, "mrhs_gshift") == 0) {
       DRX_REQUIRE(value >= 0 && value <= 3, "mrhs_gshift must be in [0, 3]");

@@ -31,6 +31,10 @@ struct Plan {
   bool owns_async = false;
   int device = 0;
   PanelSet *panels = nullptr;  // column-panel layout (drx_plan_set_panels)
+  // Cached column locality of the share for one colind pointer (k >= 2
+  // path choice): -1 unknown, 0 scattered, 1 local.
+  mutable const void *loc_colind = nullptr;
+  mutable int loc_local = -1;
 };
 
 // Launch shape over a plan's blocks: grid workgroups walk per_xcd * 8
@@ -73,6 +77,11 @@ void scratch_free(void *p, hipStream_t s);
 // Row-major chunks of x for k >= 2 right-hand sides: chunk width KP and the
 // bytes of all chunks; the copy kernel.
 int mrhs_kp(int64_t k, int xtype);
+// True when the share's columns are local to its rows (banded, stencil):
+// most row blocks' windows span few more columns than entries. Measured
+// once per colind pointer (one small kernel and a wait), cached in the plan;
+// false while the stream is being captured.
+bool columns_local(const Plan &p, const void *colind, hipStream_t s);
 size_t mrhs_xr_bytes(int64_t ncols, int64_t k, int xtype);
 void launch_x_rowmajor(int xtype, const void *x, int64_t ldx, int64_t ncols,
                        int64_t k, void *xr, hipStream_t s);

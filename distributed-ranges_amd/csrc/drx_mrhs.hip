@@ -62,6 +62,62 @@ void scratch_free(void *p, hipStream_t s) {
   if (p) (void)hipFreeAsync(p, s);
 }
 
+template <typename TI>
+__global__ void k_col_local(const BlockRec *rec, int64_t nblocks, const TI *colind,
+                            int *out) {
+  __shared__ int cnt[2];
+  if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t S = nblocks < 4096 ? nblocks : 4096;  // sampled blocks
+  int local = 0, seen = 0;
+  for (int64_t i = threadIdx.x; i < S; i += blockDim.x) {
+    const BlockRec b = rec[i * nblocks / S];
+    const int64_t n = b.es - b.e0;
+    if (n >= 2) {
+      const int64_t lo = static_cast<int64_t>(colind[b.e0]);
+      const int64_t hi = static_cast<int64_t>(colind[b.es - 1]);
+      const int64_t span = hi > lo ? hi - lo : lo - hi;
+      seen++;
+      if (span <= 4 * n + 64) local++;
+    }
+  }
+  atomicAdd(&cnt[0], local);
+  atomicAdd(&cnt[1], seen);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = cnt[0];
+    out[1] = cnt[1];
+  }
+}
+
+bool columns_local(const Plan &p, const void *colind, hipStream_t s) {
+  if (p.nnz == 0 || p.nblocks == 0 || colind == nullptr) return false;
+  if (p.loc_colind == colind && p.loc_local >= 0) return p.loc_local == 1;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  DRX_HIP(hipStreamIsCapturing(s, &st));
+  if (st != hipStreamCaptureStatusNone) return false;
+  int *d = static_cast<int *>(scratch_alloc(2 * sizeof(int), s));
+  int h[2] = {0, 0};
+  try {
+    if (p.itype == DRX_I32)
+      hipLaunchKernelGGL(k_col_local<int32_t>, dim3(1), dim3(256), 0, s, p.rec, p.nblocks,
+                         static_cast<const int32_t *>(colind), d);
+    else
+      hipLaunchKernelGGL(k_col_local<int64_t>, dim3(1), dim3(256), 0, s, p.rec, p.nblocks,
+                         static_cast<const int64_t *>(colind), d);
+    DRX_HIP(hipGetLastError());
+    DRX_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s));
+    DRX_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    scratch_free(d, s);
+    throw;
+  }
+  scratch_free(d, s);
+  p.loc_local = (h[1] > 0 && 10 * h[0] >= 9 * h[1]) ? 1 : 0;
+  p.loc_colind = colind;
+  return p.loc_local == 1;
+}
+
 int mrhs_kp(int64_t k, int xtype) {
   if (k <= 2) return 2;
   if (k <= 4) return 4;

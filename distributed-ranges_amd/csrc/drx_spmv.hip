@@ -140,7 +140,15 @@ static void run_spmv_impl(const Plan &p, int vtype, const void *colind,
   a.xbytes = bufx ? static_cast<uint32_t>(xb) : 0;
   a.ncols = p.ncols;
   if constexpr (MODE != kRowMap) {
-    if (k >= 2 && tuned_mrhs()) {
+    // Row-major chunks of x for k >= 2, except up to 4 right-hand sides over
+    // short rows with local columns (stencils): their column gathers are
+    // already coalesced and the copy of x is large next to the matrix
+    // (2-D Laplacian FP64, k = 2 / 3 / 4: 3009 / 2891 / 2811 GB/s through
+    // column gathers vs 2151 / 2245 / 2572 through chunks; a +-16 band, 33
+    // entries per row, is faster through chunks at every k).
+    const int mr = tuned_mrhs();
+    const bool stencil = k <= 4 && p.nnz <= 8 * p.nrows;
+    if (k >= 2 && mr != 0 && (mr == 2 || !stencil || !columns_local(p, colind, s))) {
       run_mrhs(p, a, MODE, vtype, xtype, ytype, s);
       return;
     }
@@ -169,7 +177,7 @@ void run_spmv(const Plan &p, int vtype, const void *colind, const void *vals,
   // Column panels serve k = 1; with k >= 2 the row-major x chunks are KP
   // times wider than a panel's L2-sized slice, and the product runs over
   // the share's own arrays.
-  if (p.panels && (k == 1 || !tuned_mrhs())) {
+  if (p.panels && (k == 1 || tuned_mrhs() == 0)) {
     run_spmv_panels(p, vtype, colind, vals, xtype, x, ldx, ytype, y, ldy, k,
                     alpha, beta, s);
     return;

@@ -69,10 +69,14 @@ const char *drx_last_error(void);
 /* Process-wide tuning knobs. No counterpart in the reference.
  *   "block_nnz"   default entries per row block of new plans (0 restores the
  *                 built-in default; env DRX_BLOCK_NNZ seeds it).
- *   "mrhs"        k >= 2: 1 (default) copies x into row-major chunks on the
- *                 call's stream (scratch from a per-device pool the library
- *                 keeps) and gathers one contiguous vector per entry; 0
- *                 gathers the k columns of x directly (no scratch).
+ *   "mrhs"        k >= 2: 2 copies x into row-major chunks on the call's
+ *                 stream (scratch from a per-device pool the library keeps)
+ *                 and gathers one contiguous vector per entry; 0 gathers the
+ *                 k columns of x directly (no scratch); 1 (default) chunks
+ *                 except for k <= 4 over short rows (<= 8 entries on
+ *                 average) with local columns (stencils; locality detected
+ *                 once per colind pointer with one small kernel and a wait
+ *                 on the stream).
  *   "mrhs_gshift" lane-group shift of the k >= 2 kernel, 0..3 (default 1).
  *   "vec"         entries per lane of the streaming loads for <= 4-byte
  *                 values with 32-bit indices: 0 (default) two for 16-bit

@@ -163,12 +163,12 @@ def _ragged_with_long_rows(m, n, seed, long_rows=(7, 700), long_len=9000):
     return rp, ci, v
 
 
-@pytest.mark.parametrize("mrhs", [1, 0])
-@pytest.mark.parametrize("k", [2, 4, 5, 8, 9, 16, 20, 33])
+@pytest.mark.parametrize("mrhs", [2, 0])
+@pytest.mark.parametrize("k", [2, 3, 4, 5, 8, 9, 16, 20, 33])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_multi_rhs_paths(orc, mrhs, k, dtype):
-    """k right-hand sides through row-major chunks of x (mrhs=1: chunk widths
-    4, 8, 16 and several chunks for k > 16) and through direct column
+    """k right-hand sides through row-major chunks of x (mrhs=2: chunk widths
+    2, 4, 8, 16 and several chunks for k > 16) and through direct column
     gathers (mrhs=0), with empty rows and rows longer than every window;
     alpha/beta and padded leading dimensions."""
     rp, ci, v = _ragged_with_long_rows(1500, 10000, k)
@@ -591,3 +591,23 @@ def test_gather_probe():
     assert torch.equal(drx.gen.gather(idx, xd), xd[idx.long()])
     xh = x.to(torch.bfloat16)
     assert torch.equal(drx.gen.gather(idx, xh), xh[idx.long()])
+
+
+@pytest.mark.parametrize("k", [2, 4, 5])
+def test_multi_rhs_path_choice_by_column_locality(k):
+    """The default (mrhs=1) takes column gathers for k <= 4 over short rows
+    with local columns (a 2-D Laplacian) and row-major chunks otherwise (a
+    +-16 band, a random matrix, k = 5): its bits equal the forced path's."""
+    lap = drx.gen.laplacian(2, 150, vdtype=torch.float64)
+    band = drx.gen.band(20000, 16, 16, vdtype=torch.float64, seed=5)
+    rnd = drx.gen.uniform_rows_csr(20000, 20000, 3, 12, seed=6, vdtype=torch.float64)
+    for A, cols in ((lap, k <= 4), (band, False), (rnd, False)):
+        n = A.shape[1]
+        x = drx.gen.uniform(k * n, torch.float64, -1, 1, seed=k).reshape(k, -1)
+        y_auto = A.matvec(x)
+        with drx.tuned(mrhs=0):
+            y_cols = A.matvec(x)
+        with drx.tuned(mrhs=2):
+            y_chunks = A.matvec(x)
+        torch.cuda.synchronize()
+        assert torch.equal(y_auto, y_cols if cols else y_chunks), (A.shape, k)
