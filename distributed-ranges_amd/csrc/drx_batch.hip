@@ -72,6 +72,7 @@ struct Batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool mrhs = false;            // k >= 2 through row-major chunks of x
+  int kp = 0;                   // their width, fixed when the batch is made
   void *xr_all = nullptr;       // all items' chunks
   XrJob *d_jobs = nullptr;
   int64_t njobs = 0, xr_threads = 0;
@@ -248,7 +249,7 @@ template <typename TV, typename TX, typename TY, typename TA, typename TI>
 static void pick_kernel_t(const Batch &b, const void **fn, int *wg) {
   const int64_t cap = 2 * b.block_nnz;
   if (b.mrhs) {
-    const int kp = mrhs_kp(b.k, b.xtype);
+    const int kp = b.kp;
     if (kp == 2)
       pick_mrhs_kp<2, TV, TX, TY, TA, TI>(b, fn, wg);
     else if (kp == 4)
@@ -300,13 +301,13 @@ static void free_chunks(Batch &b) {
 // k >= 2: one buffer for all items' row-major chunks and the copy jobs.
 static void make_chunks(Batch &b, hipStream_t s) {
   const int64_t n = static_cast<int64_t>(b.items.size());
-  const int kp = mrhs_kp(b.k, b.xtype);
+  const int kp = b.kp;
   const int64_t nchunk = (b.k + kp - 1) / kp;
   std::vector<size_t> at(n);
   size_t bytes = 0;
   for (int64_t i = 0; i < n; i++) {
     at[i] = bytes;
-    bytes += (mrhs_xr_bytes(b.items[i].src->ncols, b.k, b.xtype) + 255) & ~size_t(255);
+    bytes += (mrhs_xr_bytes(b.items[i].src->ncols, b.k, kp, b.xtype) + 255) & ~size_t(255);
   }
   DRX_HIP(hipMalloc(&b.xr_all, std::max<size_t>(bytes, 256)));
   std::vector<XrJob> jobs;
@@ -331,7 +332,7 @@ static void make_chunks(Batch &b, hipStream_t s) {
 
 static void launch_chunks(const Batch &b, hipStream_t s) {
   if (b.xr_threads == 0) return;
-  const int kp = mrhs_kp(b.k, b.xtype);
+  const int kp = b.kp;
   const dim3 g(static_cast<unsigned>(std::min<int64_t>((b.xr_threads + 255) / 256, 1 << 16)));
   const size_t xs = dtype_size(b.xtype);
 #define DRX_BXR(KP, T)                                                         \
@@ -503,6 +504,7 @@ int drx_batch_create(drx_batch *out, int64_t nitems, const drx_batch_item *items
       b->ytype = ytype;
       b->k = k;
       b->mrhs = k >= 2 && tuned_mrhs() != 0;
+      b->kp = b->mrhs ? mrhs_kp(k, xtype, tuned_mrhs_kp_max() == 8) : 0;
       b->alpha = alpha;
       b->beta = beta;
       int dev = 0;

@@ -26,6 +26,8 @@ static std::atomic<int64_t> g_grid_cap{0};
 
 int tuned_mrhs() { return g_mrhs.load(); }
 int tuned_mrhs_gshift() { return g_mrhs_gshift.load(); }
+static std::atomic<int> g_mrhs_kp_max{0};
+int tuned_mrhs_kp_max() { return g_mrhs_kp_max.load(); }
 int tuned_vec() { return g_vec.load(); }
 int64_t tuned_grid_cap() { return g_grid_cap.load(); }
 
@@ -337,6 +339,9 @@ int drx_tune_set(const char *key, int64_t value) {
     } else if (std::strcmp(key, "mrhs_gshift") == 0) {
       DRX_REQUIRE(value >= 0 && value <= 3, "mrhs_gshift must be in [0, 3]");
       g_mrhs_gshift.store(static_cast<int>(value));
+    } else if (std::strcmp(key, "mrhs_kp_max") == 0) {
+      DRX_REQUIRE(value == 0 || value == 8 || value == 16, "mrhs_kp_max must be 0, 8 or 16");
+      g_mrhs_kp_max.store(static_cast<int>(value));
     } else if (std::strcmp(key, "vec") == 0) {
       DRX_REQUIRE(value >= 0 && value <= 2, "vec must be 0, 1 or 2");
       g_vec.store(static_cast<int>(value));
@@ -360,6 +365,8 @@ int drx_tune_get(const char *key, int64_t *value) {
       *value = g_mrhs_gshift.load();
     else if (std::strcmp(key, "vec") == 0)
       *value = g_vec.load();
+    else if (std::strcmp(key, "mrhs_kp_max") == 0)
+      *value = g_mrhs_kp_max.load();
     else if (std::strcmp(key, "grid_cap") == 0)
       *value = g_grid_cap.load();
     else

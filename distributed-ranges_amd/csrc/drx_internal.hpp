@@ -32,7 +32,8 @@ struct Plan {
   int device = 0;
   PanelSet *panels = nullptr;  // column-panel layout (drx_plan_set_panels)
   // Cached column locality of the share for one colind pointer (k >= 2
-  // path choice): -1 unknown, 0 scattered, 1 local.
+  // path choice): -1 unknown, else bit 0 = tight (row blocks span about as
+  // many columns as entries), bit 1 = loose (within ncols / 16).
   mutable const void *loc_colind = nullptr;
   mutable int loc_local = -1;
 };
@@ -63,6 +64,7 @@ int64_t default_block_nnz(int itype);
 // the k columns of x directly; lane-group shift of the row-major kernel.
 int tuned_mrhs();
 int tuned_mrhs_gshift();
+int tuned_mrhs_kp_max();  // widest chunk for 4-byte x: 0 by locality, 8, 16
 // Entries per lane of the streaming loads: 0 = by type and mode, 1, or 2
 // where eligible;
 // the most workgroups of a per-share launch (0 = one per row block, up to
@@ -76,15 +78,19 @@ void scratch_free(void *p, hipStream_t s);
 
 // Row-major chunks of x for k >= 2 right-hand sides: chunk width KP and the
 // bytes of all chunks; the copy kernel.
-int mrhs_kp(int64_t k, int xtype);
-// True when the share's columns are local to its rows (banded, stencil):
-// most row blocks' windows span few more columns than entries. Measured
-// once per colind pointer (one small kernel and a wait), cached in the plan;
-// false while the stream is being captured.
-bool columns_local(const Plan &p, const void *colind, hipStream_t s);
-size_t mrhs_xr_bytes(int64_t ncols, int64_t k, int xtype);
+// Chunk width for k right-hand sides; `loose_local` allows 8-wide chunks
+// for k > 8 (fewer running sums, more waves: faster when the gathers of
+// neighbouring rows coalesce).
+int mrhs_kp(int64_t k, int xtype, bool loose_local = false);
+// Column locality of the share: bit 0 when most row blocks' windows span few
+// more columns than entries (banded, 2-D stencils), bit 1 when they span at
+// most ncols / 16 (3-D stencils too). Measured once per colind pointer (one
+// small kernel and a wait), cached in the plan; 0 while the stream is being
+// captured.
+int column_locality(const Plan &p, const void *colind, hipStream_t s);
+size_t mrhs_xr_bytes(int64_t ncols, int64_t k, int kp, int xtype);
 void launch_x_rowmajor(int xtype, const void *x, int64_t ldx, int64_t ncols,
-                       int64_t k, void *xr, hipStream_t s);
+                       int64_t k, int kp, void *xr, hipStream_t s);
 
 Plan *make_plan(int itype, const void *rowptr, int64_t base, int64_t nrows,
                 int64_t ncols, int64_t nnz, int sent, int64_t block_nnz,

@@ -148,7 +148,8 @@ static void run_spmv_impl(const Plan &p, int vtype, const void *colind,
     // entries per row, is faster through chunks at every k).
     const int mr = tuned_mrhs();
     const bool stencil = k <= 4 && p.nnz <= 8 * p.nrows;
-    if (k >= 2 && mr != 0 && (mr == 2 || !stencil || !columns_local(p, colind, s))) {
+    if (k >= 2 && mr != 0 &&
+        (mr == 2 || !stencil || (column_locality(p, colind, s) & 1) == 0)) {
       run_mrhs(p, a, MODE, vtype, xtype, ytype, s);
       return;
     }

@@ -78,6 +78,10 @@ const char *drx_last_error(void);
  *                 once per colind pointer with one small kernel and a wait
  *                 on the stream).
  *   "mrhs_gshift" lane-group shift of the k >= 2 kernel, 0..3 (default 1).
+ *   "mrhs_kp_max" widest row-major chunk for 4-byte x: 16, 8, or 0 (default:
+ *                 8 for k > 8 over loosely local columns -- stencils,
+ *                 bands -- else 16; batches use 16). Fixed per call, and per
+ *                 batch when it is created.
  *   "vec"         entries per lane of the streaming loads for <= 4-byte
  *                 values with 32-bit indices: 0 (default) two for 16-bit
  *                 values and for column panels, one otherwise; 1 always

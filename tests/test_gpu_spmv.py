@@ -611,3 +611,22 @@ def test_multi_rhs_path_choice_by_column_locality(k):
             y_chunks = A.matvec(x)
         torch.cuda.synchronize()
         assert torch.equal(y_auto, y_cols if cols else y_chunks), (A.shape, k)
+
+
+def test_multi_rhs_chunk_width_by_locality(orc):
+    """k > 8 FP32: 8-wide chunks over loosely local columns (a 3-D stencil),
+    16-wide over random columns; the default's bits equal the forced width's,
+    and both are within tolerance of the oracle."""
+    lap = drx.gen.laplacian(3, 24, vdtype=torch.float32)
+    rnd = drx.gen.uniform_rows_csr(14000, 14000, 3, 30, seed=8)
+    for A, kp in ((lap, 8), (rnd, 16)):
+        k = 12
+        x = drx.gen.uniform(k * A.shape[1], torch.float32, -1, 1, seed=9).reshape(k, -1)
+        y_auto = A.matvec(x)
+        with drx.tuned(mrhs_kp_max=kp):
+            y_forced = A.matvec(x)
+        torch.cuda.synchronize()
+        assert torch.equal(y_auto, y_forced), kp
+        rp, ci, v = A.rowptr.cpu().numpy(), A.colind.cpu().numpy(), A.vals.cpu().numpy()
+        ys, absy = orc.spmv(rp, ci, v, x.cpu().numpy())
+        assert componentwise_err(y_auto.cpu().numpy(), ys, absy) < 1e-5
