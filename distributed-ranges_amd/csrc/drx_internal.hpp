@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 
 namespace drx {
@@ -34,8 +35,11 @@ struct Plan {
   // Cached column locality of the share for one colind pointer (k >= 2
   // path choice): -1 unknown, else bit 0 = tight (row blocks span about as
   // many columns as entries), bit 1 = loose (within ncols / 16).
-  mutable const void *loc_colind = nullptr;
-  mutable int loc_local = -1;
+  // Atomics: calls sharing the plan from several threads read and write the
+  // cache without a lock. A torn read only picks the other, equally correct,
+  // k >= 2 path once.
+  mutable std::atomic<const void *> loc_colind{nullptr};
+  mutable std::atomic<int> loc_local{-1};
 };
 
 // Launch shape over a plan's blocks: grid workgroups walk per_xcd * 8

@@ -98,7 +98,10 @@ __global__ void k_col_local(const BlockRec *rec, int64_t nblocks, int64_t ncols,
 
 int column_locality(const Plan &p, const void *colind, hipStream_t s) {
   if (p.nnz == 0 || p.nblocks == 0 || colind == nullptr) return 0;
-  if (p.loc_colind == colind && p.loc_local >= 0) return p.loc_local;
+  {
+    const int cached = p.loc_local.load(std::memory_order_acquire);
+    if (cached >= 0 && p.loc_colind.load(std::memory_order_acquire) == colind) return cached;
+  }
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
   DRX_HIP(hipStreamIsCapturing(s, &st));
   if (st != hipStreamCaptureStatusNone) return 0;
@@ -122,8 +125,10 @@ int column_locality(const Plan &p, const void *colind, hipStream_t s) {
   int loc = 0;
   if (h[2] > 0 && 10 * h[0] >= 9 * h[2]) loc |= 1;
   if (h[2] > 0 && 10 * h[1] >= 9 * h[2]) loc |= 2;
-  p.loc_local = loc;
-  p.loc_colind = colind;
+  // Invalidate, then publish pointer and value.
+  p.loc_local.store(-1, std::memory_order_release);
+  p.loc_colind.store(colind, std::memory_order_release);
+  p.loc_local.store(loc, std::memory_order_release);
   return loc;
 }
 
