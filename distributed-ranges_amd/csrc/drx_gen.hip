@@ -22,6 +22,8 @@ static std::atomic<int64_t> g_block_nnz{0};
 static std::atomic<int> g_mrhs{1};
 static std::atomic<int> g_mrhs_gshift{1};
 static std::atomic<int> g_vec{0};
+static std::atomic<int> g_wide{1};
+int tuned_wide() { return g_wide.load(); }
 static std::atomic<int64_t> g_grid_cap{0};
 
 int tuned_mrhs() { return g_mrhs.load(); }
@@ -342,6 +344,9 @@ int drx_tune_set(const char *key, int64_t value) {
     } else if (std::strcmp(key, "mrhs_kp_max") == 0) {
       DRX_REQUIRE(value == 0 || value == 8 || value == 16, "mrhs_kp_max must be 0, 8 or 16");
       g_mrhs_kp_max.store(static_cast<int>(value));
+    } else if (std::strcmp(key, "wide") == 0) {
+      DRX_REQUIRE(value == 0 || value == 1, "wide must be 0 or 1");
+      g_wide.store(static_cast<int>(value));
     } else if (std::strcmp(key, "vec") == 0) {
       DRX_REQUIRE(value >= 0 && value <= 2, "vec must be 0, 1 or 2");
       g_vec.store(static_cast<int>(value));
@@ -365,6 +370,8 @@ int drx_tune_get(const char *key, int64_t *value) {
       *value = g_mrhs_gshift.load();
     else if (std::strcmp(key, "vec") == 0)
       *value = g_vec.load();
+    else if (std::strcmp(key, "wide") == 0)
+      *value = g_wide.load();
     else if (std::strcmp(key, "mrhs_kp_max") == 0)
       *value = g_mrhs_kp_max.load();
     else if (std::strcmp(key, "grid_cap") == 0)

@@ -29,6 +29,10 @@ struct Plan {
   int64_t cap = 2048;  // LDS window (entries) = 2 * block_nnz
   int64_t nblocks = 0;
   BlockRec *rec = nullptr;  // device, nblocks + 1
+  // Blocks of 2048 entries (window 4096) for plans of 1024-entry blocks:
+  // the forward kernel's layout for 4-byte values (16 entries per lane).
+  BlockRec *rec_wide = nullptr;
+  int64_t nblocks_wide = 0;
   bool owns_async = false;
   int device = 0;
   PanelSet *panels = nullptr;  // column-panel layout (drx_plan_set_panels)
@@ -74,6 +78,9 @@ int tuned_mrhs_kp_max();  // widest chunk for 4-byte x: 0 by locality, 8, 16
 // the most workgroups of a per-share launch (0 = one per row block, up to
 // 2^22), which then loop over blocks.
 int tuned_vec();
+// 1 (default): 4-byte values with 32-bit indices use the plan's 2048-entry
+// blocks (window 4096, 16 entries per lane); 0: the plan's own blocks.
+int tuned_wide();
 int64_t tuned_grid_cap();
 
 // Stream-ordered scratch from the library's per-device pool.

@@ -72,29 +72,40 @@ int64_t default_block_nnz(int itype) {
   return 1024;
 }
 
-static void build_records(Plan &p, hipStream_t s, bool async_alloc) {
-  p.nblocks = p.nnz / p.block_nnz + 1;
-  p.cap = 2 * p.block_nnz;
-  const size_t bytes = static_cast<size_t>(p.nblocks + 1) * sizeof(BlockRec);
+// Records of blocks of `nb` entries (window 2 nb) into a new array.
+static BlockRec *make_records(const Plan &p, int64_t nb, int64_t *nblocks, hipStream_t s,
+                              bool async_alloc) {
+  *nblocks = p.nnz / nb + 1;
+  const size_t bytes = static_cast<size_t>(*nblocks + 1) * sizeof(BlockRec);
+  BlockRec *rec = nullptr;
   if (async_alloc)
-    DRX_HIP(hipMallocAsync(reinterpret_cast<void **>(&p.rec), bytes, s));
+    DRX_HIP(hipMallocAsync(reinterpret_cast<void **>(&rec), bytes, s));
   else
-    DRX_HIP(hipMalloc(reinterpret_cast<void **>(&p.rec), bytes));
-  p.owns_async = async_alloc;
+    DRX_HIP(hipMalloc(reinterpret_cast<void **>(&rec), bytes));
   const int tpb = 256;
-  const int64_t n = p.nblocks + 1;
+  const int64_t n = *nblocks + 1;
   const dim3 grid(static_cast<unsigned>((n + tpb - 1) / tpb));
   if (p.itype == DRX_I32)
     hipLaunchKernelGGL(k_plan_records<int32_t>, grid, dim3(tpb), 0, s,
                        static_cast<const int32_t *>(p.rowptr), p.base, p.nrows,
-                       p.nnz, p.has_sentinel, p.block_nnz, p.cap, p.nblocks,
-                       p.rec);
+                       p.nnz, p.has_sentinel, nb, 2 * nb, *nblocks, rec);
   else
     hipLaunchKernelGGL(k_plan_records<int64_t>, grid, dim3(tpb), 0, s,
                        static_cast<const int64_t *>(p.rowptr), p.base, p.nrows,
-                       p.nnz, p.has_sentinel, p.block_nnz, p.cap, p.nblocks,
-                       p.rec);
+                       p.nnz, p.has_sentinel, nb, 2 * nb, *nblocks, rec);
   DRX_HIP(hipGetLastError());
+  return rec;
+}
+
+// The plan's records, and the records of twice as large blocks (window
+// 4096) that the forward kernel uses for 4-byte values when the plan's
+// block is 1024 entries.
+static void build_records(Plan &p, hipStream_t s, bool async_alloc) {
+  p.cap = 2 * p.block_nnz;
+  p.owns_async = async_alloc;
+  p.rec = make_records(p, p.block_nnz, &p.nblocks, s, async_alloc);
+  if (p.block_nnz == 1024)
+    p.rec_wide = make_records(p, 2048, &p.nblocks_wide, s, async_alloc);
 }
 
 static void validate_plan_args(int itype, const void *rowptr, int64_t nrows,
@@ -166,11 +177,12 @@ Plan *make_plan(int itype, const void *rowptr, int64_t base, int64_t nrows,
 void free_plan(Plan *p, hipStream_t s) {
   if (!p) return;
   if (p->panels) free_panels(p->panels);
-  if (p->rec) {
+  for (BlockRec *r : {p->rec, p->rec_wide}) {
+    if (!r) continue;
     if (p->owns_async)
-      (void)hipFreeAsync(p->rec, s);
+      (void)hipFreeAsync(r, s);
     else
-      (void)hipFree(p->rec);
+      (void)hipFree(r);
   }
   delete p;
 }

@@ -35,20 +35,33 @@ Grid grid_for(int64_t nblocks) {
   return g;
 }
 
+// Window 4096 on 256 lanes (16 entries per lane) for 4-byte values and
+// indices in plain mode (the permuted-values variant spills registers):
+// measured faster than 512 lanes x 8 (band +-16 FP32: 3544 vs 3006
+// GB/s) and than 1024-entry blocks (3-D Laplacian FP32: 3369 vs 3094).
+template <int MODE, typename TV, typename TA, typename TI>
+constexpr bool wide_window() {
+  return MODE == kPlain && sizeof(TV) == 4 && sizeof(TA) == 4 && sizeof(TI) == 4;
+}
+
 template <int MODE, bool BUFX, typename TV, typename TX, typename TY,
           typename TA, typename TI, int VW>
-static void launch_fwd_vw(const Plan &p, const FwdArgs &a, hipStream_t s) {
+static void launch_fwd_vw(int64_t cap, const FwdArgs &a, hipStream_t s) {
   const dim3 grid(static_cast<unsigned>(a.grid));
-  if (p.cap <= 1024)
+  if (cap <= 1024)
     hipLaunchKernelGGL((k_spmv_fwd<256, 1024, MODE, BUFX, TV, TX, TY, TA, TI, VW>),
                        grid, dim3(256), 0, s, a);
-  else if (p.cap <= 2048)
+  else if (cap <= 2048)
     hipLaunchKernelGGL((k_spmv_fwd<256, 2048, MODE, BUFX, TV, TX, TY, TA, TI, VW>),
                        grid, dim3(256), 0, s, a);
-  else if (p.cap <= 4096)
-    hipLaunchKernelGGL((k_spmv_fwd<512, 4096, MODE, BUFX, TV, TX, TY, TA, TI, VW>),
-                       grid, dim3(512), 0, s, a);
-  else
+  else if (cap <= 4096) {
+    if constexpr (wide_window<MODE, TV, TA, TI>())
+      hipLaunchKernelGGL((k_spmv_fwd<256, 4096, MODE, BUFX, TV, TX, TY, TA, TI, VW>),
+                         grid, dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_spmv_fwd<512, 4096, MODE, BUFX, TV, TX, TY, TA, TI, VW>),
+                         grid, dim3(512), 0, s, a);
+  } else
     hipLaunchKernelGGL((k_spmv_fwd<1024, 8192, MODE, BUFX, TV, TX, TY, TA, TI, VW>),
                        grid, dim3(1024), 0, s, a);
 }
@@ -59,7 +72,19 @@ static void launch_fwd_vw(const Plan &p, const FwdArgs &a, hipStream_t s) {
 // it measured faster), 1 never pairs, 2 pairs wherever eligible.
 template <int MODE, bool BUFX, typename TV, typename TX, typename TY,
           typename TA, typename TI>
-static void launch_fwd(const Plan &p, const FwdArgs &a, hipStream_t s) {
+static void launch_fwd(const Plan &p, FwdArgs a, hipStream_t s) {
+  int64_t cap = p.cap;
+  if constexpr (wide_window<MODE, TV, TA, TI>()) {
+    // the plan's 2048-entry blocks (window 4096), unless the knob says no
+    if (p.rec_wide != nullptr && tuned_wide() != 0) {
+      a.rec = p.rec_wide;
+      a.nblocks = p.nblocks_wide;
+      const Grid g = grid_for(p.nblocks_wide);
+      a.per_xcd = g.per_xcd;
+      a.grid = g.grid;
+      cap = 4096;
+    }
+  }
   if constexpr (MODE != kPerm && sizeof(TI) == 4 && sizeof(TV) <= 4) {
     const bool aligned =
         reinterpret_cast<uintptr_t>(a.vals) % (2 * sizeof(TV)) == 0 &&
@@ -67,11 +92,11 @@ static void launch_fwd(const Plan &p, const FwdArgs &a, hipStream_t s) {
     const int vec = tuned_vec();
     const bool want = vec == 2 || (vec == 0 && (sizeof(TV) == 2 || MODE == kRowMap));
     if (aligned && want) {
-      launch_fwd_vw<MODE, BUFX, TV, TX, TY, TA, TI, 2>(p, a, s);
+      launch_fwd_vw<MODE, BUFX, TV, TX, TY, TA, TI, 2>(cap, a, s);
       return;
     }
   }
-  launch_fwd_vw<MODE, BUFX, TV, TX, TY, TA, TI, 1>(p, a, s);
+  launch_fwd_vw<MODE, BUFX, TV, TX, TY, TA, TI, 1>(cap, a, s);
 }
 
 template <int MODE, typename TV, typename TX, typename TY, typename TA>

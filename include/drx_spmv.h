@@ -87,6 +87,9 @@ const char *drx_last_error(void);
  *                 values and for column panels, one otherwise; 1 always
  *                 one; 2 two wherever eligible (pairs need vals and colind
  *                 aligned to a pair of entries; others use one).
+ *   "wide"        1 (default): FP32 values with 32-bit indices stream
+ *                 2048-entry blocks (16 per lane) of plans whose block_nnz
+ *                 is the default 1024; 0: the plan's own blocks.
  *   "grid_cap"    most workgroups of a per-share launch (0 = one per row
  *                 block); capped launches loop over the blocks. */
 int drx_tune_set(const char *key, int64_t value);

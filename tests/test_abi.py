@@ -61,7 +61,7 @@ def test_argument_validation_without_gpu():
     assert drx.tune_get("mrhs") == 1 and drx.tune_get("mrhs_gshift") == 1
     assert drx.tune_get("vec") == 0 and drx.tune_get("grid_cap") == 0
     assert L.drx_tune_set(b"vec", 3) == drx._lib.DRX_ERR_INVALID
-    assert drx.tune_get("mrhs_kp_max") == 0
+    assert drx.tune_get("mrhs_kp_max") == 0 and drx.tune_get("wide") == 1
     assert L.drx_tune_set(b"mrhs_kp_max", 4) == drx._lib.DRX_ERR_INVALID
     assert L.drx_tune_set(b"mrhs", 3) == drx._lib.DRX_ERR_INVALID
     assert L.drx_tune_set(b"mrhs_gshift", 4) == drx._lib.DRX_ERR_INVALID

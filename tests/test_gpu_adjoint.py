@@ -190,8 +190,10 @@ def test_autograd_gradcheck_small():
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_adjoint_with_stored_transposed_values(orc, dtype):
     """Values stored in CSC order give the same adjoint, bit for bit, as
-    reading them through the permutation; changing the values in place
-    (new tensor version) falls back to the permutation."""
+    reading them through the permutation (on the same 1024-entry blocks:
+    the stored-values FP32 path otherwise streams 2048-entry blocks, checked
+    against the oracle); changing the values in place (new tensor version)
+    falls back to the permutation."""
     A = rand_csr(5000, 4000, 0.002, 21)
     v = A.data.astype(np.float64 if dtype == torch.float64 else np.float32)
     Ad = drx.from_host(A.indptr, A.indices, v, A.shape)
@@ -199,9 +201,14 @@ def test_adjoint_with_stored_transposed_values(orc, dtype):
     via_perm = Ad.rmatvec(dy)
     Ad.set_transposed_values()
     assert Ad.stored_transposed_values(Ad.vals)
-    stored = Ad.rmatvec(dy)
+    with drx.tuned(wide=0):
+        stored = Ad.rmatvec(dy)
+    stored_wide = Ad.rmatvec(dy)
     torch.cuda.synchronize()
     assert torch.equal(stored, via_perm)
+    ref, absr = orc.spmv_t(A.indptr, A.indices, v, dy.cpu().numpy(), A.shape[1])
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    assert componentwise_err(stored_wide.cpu().numpy(), ref, absr) < tol
     # several right-hand sides: both value sources, bit for bit
     dyk = torch.as_tensor(np.random.default_rng(3).uniform(-1, 1, (9, A.shape[0])).astype(v.dtype)).to(DEV)
     stored_k = Ad.rmatvec(dyk)

@@ -443,7 +443,8 @@ def _batch_mats(nmat, seed, rows=(500, 4000), lens=(3, 64)):
 @pytest.mark.parametrize("k", [1, 4, 17])
 def test_batch_matches_single_calls_and_oracle(orc, k):
     """Forward + adjoint items of uneven matrices in one persistent launch:
-    bit-identical to the single calls at the same block size and lane-group
+    bit-identical to the single calls at the same block size (1024 entries:
+    the single FP32 calls' 2048-entry blocks turned off) and lane-group
     thresholds, and within tolerance of the oracle."""
     mats = _batch_mats(9, 3 + k)
     def vec(n, s):
@@ -460,8 +461,9 @@ def test_batch_matches_single_calls_and_oracle(orc, k):
     b.run()
     torch.cuda.synchronize()
     for A, x, y, d, dx in zip(mats, xs, ys, dys, dxs):
-        assert torch.equal(y, A.matvec(x))
-        assert torch.equal(dx, A.rmatvec(d))
+        with drx.tuned(wide=0):  # single calls on the batch's 1024-entry blocks
+            assert torch.equal(y, A.matvec(x))
+            assert torch.equal(dx, A.rmatvec(d))
         rp, ci, v = A.rowptr.cpu().numpy(), A.colind.cpu().numpy(), A.vals.cpu().numpy()
         ys_, absy = orc.spmv(rp, ci, v, x.cpu().numpy())
         assert componentwise_err(y.cpu().numpy(), ys_, absy) < 1e-5
@@ -630,3 +632,30 @@ def test_multi_rhs_chunk_width_by_locality(orc):
         rp, ci, v = A.rowptr.cpu().numpy(), A.colind.cpu().numpy(), A.vals.cpu().numpy()
         ys, absy = orc.spmv(rp, ci, v, x.cpu().numpy())
         assert componentwise_err(y_auto.cpu().numpy(), ys, absy) < 1e-5
+
+
+@pytest.mark.parametrize("wide", [1, 0])
+@pytest.mark.parametrize("lens_kind", ["ragged", "long"])
+def test_fp32_wide_blocks(orc, wide, lens_kind):
+    """FP32 with 32-bit indices streams the plan's 2048-entry blocks (window
+    4096, 16 entries per lane) or its own 1024-entry blocks: both within
+    tolerance of the oracle, with long rows (whole-workgroup path), empty
+    rows, alpha and beta."""
+    if lens_kind == "ragged":
+        rp, ci, v = _ragged_with_long_rows(6000, 30000, 11, long_rows=(), long_len=0)
+        m, n = 6000, 30000
+    else:
+        rp, ci, v = _ragged_with_long_rows(3000, 30000, 12, long_rows=(5, 2000), long_len=9000)
+        m, n = 3000, 30000
+    v = v.astype(np.float32)
+    Ad = upload(rp, ci, v, (m, n))
+    x = np.random.default_rng(3).uniform(-1, 1, n).astype(np.float32)
+    y0 = np.random.default_rng(4).uniform(-1, 1, m).astype(np.float32)
+    y = torch.as_tensor(y0).to(DEV)
+    with drx.tuned(wide=wide):
+        Ad.matvec(torch.as_tensor(x).to(DEV), y=y, alpha=1.5, beta=-0.5)
+    torch.cuda.synchronize()
+    ys, absy = orc.spmv(rp, ci, v, x)
+    expect = 1.5 * ys - 0.5 * y0.astype(np.float64)
+    den = 1.5 * absy + 0.5 * np.abs(y0)
+    assert (np.abs(y.double().cpu().numpy() - expect) <= 1e-5 * den + 1e-300).all()
