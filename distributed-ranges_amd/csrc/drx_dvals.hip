@@ -299,6 +299,22 @@ void run_dvals(const Plan &p, const void *colind, int xtype, const void *x,
       hipLaunchKernelGGL((k_dvals<WG, CAP, TX, TD, TA, int64_t>), grid,        \
                          dim3(WG), 0, s, a);                                   \
   } while (0)
+  // FP32 with 32-bit indices and rows of >= 16 entries on average: the
+  // plan's 2048-entry blocks, 16 per lane (band +-16: 2575 vs 2331 GB/s; the
+  // 7-entry rows of a 3-D Laplacian run slower that way, 2036 vs 2303)
+  const bool wide = xtype == DRX_F32 && dvtype == DRX_F32 && i32 && p.rec_wide != nullptr &&
+                    tuned_wide() != 0 && p.nnz >= 16 * p.nrows;
+  if (wide) {
+    a.rec = p.rec_wide;
+    a.nblocks = p.nblocks_wide;
+    const Grid gw = grid_for(p.nblocks_wide);
+    a.per_xcd = gw.per_xcd;
+    a.grid = gw.grid;
+    hipLaunchKernelGGL((k_dvals<256, 4096, float, float, float, int32_t>),
+                       dim3(static_cast<unsigned>(a.grid)), dim3(256), 0, s, a);
+    DRX_HIP(hipGetLastError());
+    return;
+  }
 #define DRX_DV(TX, TD, TA)                                                     \
   do {                                                                         \
     if (p.cap <= 1024)                                                         \

@@ -11,7 +11,11 @@ from .core import DTYPE_CODE, CsrMatrix, _cols, _p, _s
 
 class Batch:
     """forward: [(A, x, y)], adjoint: [(A, dy, dx)] with CsrMatrix A and
-    device tensors; x/y are (n,) or (k, n). All items share k, alpha, beta."""
+    device tensors; x/y are (n,) or (k, n). All items share k, alpha, beta.
+    An adjoint item whose matrix holds its current values in CSC order
+    (CsrMatrix.set_transposed_values) streams that copy; values changed
+    after the batch is made then need set_transposed_values again (the
+    batch reads the copy), others read A.vals through the permutation."""
 
     def __init__(self, forward=(), adjoint=(), k=1, alpha=1.0, beta=0.0, stream=None):
         forward, adjoint = list(forward), list(adjoint)

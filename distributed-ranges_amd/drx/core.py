@@ -133,6 +133,7 @@ class CsrMatrix:
                          block_nnz=block_nnz, stream=stream)
         self._t = None
         self._tvals_key = None  # (tensor id, version) of the values stored in CSC order
+        self._tvals_ref = None
 
     @property
     def vcode(self):
@@ -185,6 +186,8 @@ class CsrMatrix:
         t = self.transpose(stream)
         check(lib().drx_transpose_set_values(t.handle, DTYPE_CODE[vals.dtype], _p(vals),
                                              _s(stream)))
+        # the tensor itself is kept, so its id cannot be reused by another
+        self._tvals_ref = vals
         self._tvals_key = (id(vals), vals._version, vals.dtype)
 
     def stored_transposed_values(self, vals):

@@ -220,3 +220,22 @@ def test_adjoint_with_stored_transposed_values(orc, dtype):
     dx2 = Ad.rmatvec(dy)
     ref, absr = orc.spmv_t(A.indptr, A.indices, v * 2, dy.cpu().numpy(), A.shape[1])
     assert componentwise_err(dx2.cpu().numpy(), ref, absr) < (1e-12 if dtype == torch.float64 else 1e-5)
+
+
+@pytest.mark.parametrize("wide", [1, 0])
+def test_dvals_fp32_wide_blocks(orc, wide):
+    """FP32 dvals over rows of >= 16 entries stream the plan's 2048-entry
+    blocks (or the plan's own with wide = 0), with long and empty rows."""
+    rng = np.random.default_rng(40 + wide)
+    lens = [0] * 50 + list(rng.integers(10, 60, 3000)) + [9000, 0, 20000]
+    n = 30000
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    ci = np.concatenate([np.sort(rng.choice(n, L, replace=False)) for L in lens]).astype(np.int64)
+    v = rng.uniform(-1, 1, ci.size).astype(np.float32)
+    Ad = drx.from_host(rp, ci, v, (len(lens), n))
+    x = rng.uniform(-1, 1, n).astype(np.float32)
+    dy = rng.uniform(-1, 1, len(lens)).astype(np.float32)
+    with drx.tuned(wide=wide):
+        d = Ad.dvals(torch.as_tensor(x).to(DEV), torch.as_tensor(dy).to(DEV))
+    ref = orc.dvals(rp, ci, x, dy)
+    assert np.abs(d.cpu().numpy() - ref).max() <= 1e-6
