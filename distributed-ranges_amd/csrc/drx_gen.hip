@@ -32,6 +32,8 @@ static std::atomic<int> g_mrhs_kp_max{0};
 int tuned_mrhs_kp_max() { return g_mrhs_kp_max.load(); }
 int tuned_vec() { return g_vec.load(); }
 int64_t tuned_grid_cap() { return g_grid_cap.load(); }
+static std::atomic<int> g_gemm_tile{0};
+int tuned_gemm_tile() { return g_gemm_tile.load(); }
 
 int64_t tuned_block_nnz() {
   int64_t v = g_block_nnz.load();
@@ -353,6 +355,9 @@ int drx_tune_set(const char *key, int64_t value) {
     } else if (std::strcmp(key, "grid_cap") == 0) {
       DRX_REQUIRE(value >= 0 && value <= (int64_t(1) << 30), "grid_cap out of range");
       g_grid_cap.store(value);
+    } else if (std::strcmp(key, "gemm_tile") == 0) {
+      DRX_REQUIRE(value >= 0 && value <= 2, "gemm_tile must be 0, 1 or 2");
+      g_gemm_tile.store(static_cast<int>(value));
     } else {
       throw Error(DRX_ERR_INVALID, std::string("unknown tuning key: ") + key);
     }
@@ -376,6 +381,8 @@ int drx_tune_get(const char *key, int64_t *value) {
       *value = g_mrhs_kp_max.load();
     else if (std::strcmp(key, "grid_cap") == 0)
       *value = g_grid_cap.load();
+    else if (std::strcmp(key, "gemm_tile") == 0)
+      *value = g_gemm_tile.load();
     else
       throw Error(DRX_ERR_INVALID, std::string("unknown tuning key: ") + key);
   });

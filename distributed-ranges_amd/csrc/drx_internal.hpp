@@ -82,6 +82,8 @@ int tuned_vec();
 // blocks (window 4096, 16 entries per lane); 0: the plan's own blocks.
 int tuned_wide();
 int64_t tuned_grid_cap();
+// Dense product workgroup tile: 0 = by size, 1 = 128 x 128, 2 = 256 x 128.
+int tuned_gemm_tile();
 
 // Stream-ordered scratch from the library's per-device pool.
 void *scratch_alloc(size_t bytes, hipStream_t s);

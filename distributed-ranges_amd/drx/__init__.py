@@ -10,8 +10,9 @@ from .core import CsrMatrix, Plan, Transpose, from_host
 from . import gen
 from .batch import Batch
 from .mmio import mmread, read_csr_host
+from .dense import gemm
 from .autograd import SpMVFunction, spmv
 
 __all__ = ["BF16", "F16", "F32", "F64", "I32", "I64", "DrxError", "LIB_PATH",
            "check", "lib", "tune_get", "tune_set", "tuned", "CsrMatrix", "Plan", "Transpose", "from_host", "gen",
-           "SpMVFunction", "spmv", "Batch", "mmread", "read_csr_host"]
+           "SpMVFunction", "spmv", "Batch", "mmread", "read_csr_host", "gemm"]

@@ -35,6 +35,8 @@ SIGNATURES = {
                                 ctypes.POINTER(c_vp), ctypes.POINTER(c_vp)]),
     "drx_host_free": (None, [c_vp]),
     "drx_gather": (c_int, [c_vp, c_i64, c_int, c_vp, c_i64, c_vp, c_vp]),
+    "drx_gemm": (c_int, [c_int, c_i64, c_i64, c_i64, ctypes.c_double, c_vp, c_i64, c_vp, c_i64,
+                         ctypes.c_double, c_vp, c_i64, c_vp]),
     "drx_tune_set": (c_int, [ctypes.c_char_p, c_i64]),
     "drx_tune_get": (c_int, [ctypes.c_char_p, ctypes.POINTER(c_i64)]),
     "drx_plan_create": (c_int, [ctypes.POINTER(c_vp), c_int, c_vp, c_i64, c_i64,

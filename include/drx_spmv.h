@@ -91,7 +91,10 @@ const char *drx_last_error(void);
  *                 2048-entry blocks (16 per lane) of plans whose block_nnz
  *                 is the default 1024; 0: the plan's own blocks.
  *   "grid_cap"    most workgroups of a per-share launch (0 = one per row
- *                 block); capped launches loop over the blocks. */
+ *                 block); capped launches loop over the blocks.
+ *   "gemm_tile"   drx_gemm workgroup tile: 0 (default) 256 x 128 when that
+ *                 gives at least 512 tiles, else 128 x 128; 1 always
+ *                 128 x 128; 2 always 256 x 128. */
 int drx_tune_set(const char *key, int64_t value);
 int drx_tune_get(const char *key, int64_t *value);
 
@@ -319,6 +322,20 @@ int drx_mm_read_csr(const char *path, int one_indexed, int itype, int vtype,
                     int nthreads, int64_t *nrows, int64_t *ncols, int64_t *nnz,
                     void **rowptr, void **colind, void **vals);
 void drx_host_free(void *p);
+
+/* ---- Dense tile product ---------------------------------------------------
+ * C = alpha * A * B + beta * C for row-major A (m x k, leading dimension
+ * lda), B (k x n, ldb) and C (m x n, ldc), vtype DRX_F32 or DRX_F64, on the
+ * matrix cores. Replaces dr::sp::__detail::local_gemm
+ * (include/dr/sp/algorithms/matrix/local_gemm.hpp:45-60, oneMKL's row-major
+ * gemm called with alpha = beta = 1; :15-41 custom_gemm, c += a * b), the
+ * per-tile step of dr::sp::gemm (include/dr/sp/algorithms/matrix/
+ * gemm.hpp:12-146). beta == 0 does not read C. C must not overlap A or B.
+ * Errors: DRX_ERR_INVALID for negative sizes, a leading dimension below the
+ * row length, null operands, other vtypes. Asynchronous on `stream`. */
+int drx_gemm(int vtype, int64_t m, int64_t n, int64_t k, double alpha,
+             const void *a, int64_t lda, const void *b, int64_t ldb,
+             double beta, void *c, int64_t ldc, drx_stream_t stream);
 
 #ifdef __cplusplus
 }

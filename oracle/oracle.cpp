@@ -37,10 +37,15 @@
 //   orc_csr_transpose         scipy.sparse .T.tocsr() structure (bit-exact).
 //   orc_gemveq_cpu_bench      same output as orc_mp_gemv_eq (bit-exact) for
 //                             every P; it is the same arithmetic, threaded.
+//   orc_gemm_ref              closed form of the sp gemm example's inputs
+//                             (a = (i + k) / m, b = (k + j) / k), identity
+//                             operands, numpy's float64 matmul, alpha/beta
+//                             linearity.
 #include <algorithm>
 #include <atomic>
 #include <barrier>
 #include <chrono>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <functional>
@@ -399,6 +404,45 @@ void csr_transpose(const I *rowptr, std::int64_t base, std::int64_t nrows,
 //   gather to root (Gatherv == root copies the partials in rank order);
 //   root: fill(res, 0), boundary-row add and copies (csr_eq:185-205);
 //   fence (barrier).
+// Dense tile product, the step of dr::sp::gemm (sp gemm). custom_gemm
+// (include/dr/sp/algorithms/matrix/local_gemm.hpp:15-41) adds a[i][k] * b[k][j]
+// into c[i][j] for every (i, k, j) of row-major views; mkl_gemm (:45-60) is
+// the same product, alpha = beta = 1. Here out = alpha * sum + beta * c in
+// long double (c not read when beta == 0) and absout = |alpha| sum |a||b| +
+// |beta||c|, the scale of the rounding-error bound.
+template <typename T>
+void gemm_ref(std::int64_t m, std::int64_t n, std::int64_t k, double alpha,
+              const T *a, std::int64_t lda, const T *b, std::int64_t ldb,
+              double beta, const T *c, std::int64_t ldc, double *out,
+              double *absout) {
+  std::vector<long double> s(n), sa(n);
+  for (std::int64_t i = 0; i < m; i++) {
+    std::fill(s.begin(), s.end(), 0.0L);
+    std::fill(sa.begin(), sa.end(), 0.0L);
+    for (std::int64_t q = 0; q < k; q++) {
+      const long double av = a[i * lda + q];
+      const long double aa = av < 0 ? -av : av;
+      const T *brow = b + q * ldb;
+      for (std::int64_t j = 0; j < n; j++) {
+        const long double bv = brow[j];
+        s[j] += av * bv;
+        sa[j] += aa * (bv < 0 ? -bv : bv);
+      }
+    }
+    for (std::int64_t j = 0; j < n; j++) {
+      long double v = static_cast<long double>(alpha) * s[j];
+      long double w = std::fabs(static_cast<long double>(alpha)) * sa[j];
+      if (beta != 0) {
+        const long double cv = c[i * ldc + j];
+        v += static_cast<long double>(beta) * cv;
+        w += std::fabs(static_cast<long double>(beta)) * (cv < 0 ? -cv : cv);
+      }
+      out[i * n + j] = static_cast<double>(v);
+      if (absout) absout[i * n + j] = static_cast<double>(w);
+    }
+  }
+}
+
 template <typename T, typename I>
 double gemveq_cpu_bench(std::size_t m, std::size_t n, std::size_t nnz,
                         const I *rowptr, const I *colind, const T *vals,
@@ -658,6 +702,19 @@ void orc_eq_partition(std::size_t m, std::size_t nnz, const std::int64_t *rp,
 ORC_MP(float, f32)
 ORC_MP(double, f64)
 #undef ORC_MP
+
+void orc_gemm_ref_f32(std::int64_t m, std::int64_t n, std::int64_t k, double alpha,
+                      const float *a, std::int64_t lda, const float *b,
+                      std::int64_t ldb, double beta, const float *c,
+                      std::int64_t ldc, double *out, double *absout) {
+  gemm_ref(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, out, absout);
+}
+void orc_gemm_ref_f64(std::int64_t m, std::int64_t n, std::int64_t k, double alpha,
+                      const double *a, std::int64_t lda, const double *b,
+                      std::int64_t ldb, double beta, const double *c,
+                      std::int64_t ldc, double *out, double *absout) {
+  gemm_ref(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, out, absout);
+}
 
 void orc_csr_transpose(const std::int64_t *rp, std::int64_t base,
                        std::int64_t nrows, std::int64_t ncols,

@@ -57,6 +57,9 @@ class Oracle:
             getattr(L, nm).restype = _Z
         L.orc_gen_band_csr_f64_i64.argtypes = [_Z, _Z, _Z, _P, _P, _P]
         L.orc_gen_band_csr_f64_i64.restype = _Z
+        for t in ("f32", "f64"):
+            getattr(L, "orc_gemm_ref_" + t).argtypes = [_I, _I, _I, ctypes.c_double, _P, _I, _P,
+                                                        _I, ctypes.c_double, _P, _I, _P, _P]
 
     # -- partitions ---------------------------------------------------------
     def row_partition(self, rowptr, nnz, P):
@@ -110,6 +113,23 @@ class Oracle:
         return (res[0] if xx.ndim == 1 else res), med, times
 
     # -- extended-precision oracle -----------------------------------------
+    def gemm(self, a, b, c=None, alpha=1.0, beta=1.0):
+        """(alpha a @ b + beta c, |alpha||a||b| + |beta||c|) as float64 from
+        2-D row-major float32/float64 arrays (c None: beta = 0)."""
+        t = "f32" if a.dtype == np.float32 else "f64"
+        a = np.ascontiguousarray(a)
+        b = np.ascontiguousarray(b, dtype=a.dtype)
+        m, k = a.shape
+        n = b.shape[1]
+        if c is None:
+            c, beta = np.zeros((m, n), a.dtype), 0.0
+        c = np.ascontiguousarray(c, dtype=a.dtype)
+        out = np.zeros((m, n))
+        absout = np.zeros((m, n))
+        getattr(self.L, "orc_gemm_ref_" + t)(m, n, k, alpha, ptr(a), max(k, 1), ptr(b), max(n, 1),
+                                             beta, ptr(c), max(n, 1), ptr(out), ptr(absout))
+        return out, absout
+
     def spmv(self, rowptr, colind, vals, x, nrows=None, base=0, nnz=None,
              sentinel=True, ldx=None):
         """(y, |A||x|) as float64; x (n,) or (k, n)."""

@@ -72,3 +72,24 @@ def test_argument_validation_without_gpu():
     assert L.drx_gen_laplacian_nnz(3, 256) == 117047296
     assert L.drx_gen_band_nnz(4000000, 16, 16) == 131999728
     assert L.drx_gen_band_nnz(866, 17, 17) == 30004
+
+
+def test_gemm_argument_validation_without_gpu():
+    L = drx.lib()
+    bad = drx._lib.DRX_ERR_INVALID
+    p = ctypes.c_void_p(256)
+    # lda < k, ldb < n, ldc < n, unsupported vtype, negative size, null operand
+    assert L.drx_gemm(drx.F32, 4, 6, 5, 1.0, p, 4, p, 6, 1.0, p, 6, None) == bad
+    assert L.drx_gemm(drx.F32, 4, 6, 5, 1.0, p, 5, p, 5, 1.0, p, 6, None) == bad
+    assert L.drx_gemm(drx.F64, 4, 6, 5, 1.0, p, 5, p, 6, 1.0, p, 5, None) == bad
+    assert L.drx_gemm(drx.BF16, 4, 6, 5, 1.0, p, 5, p, 6, 1.0, p, 6, None) == bad
+    assert b"vtype" in L.drx_last_error()
+    assert L.drx_gemm(drx.F32, -1, 6, 5, 1.0, p, 5, p, 6, 1.0, p, 6, None) == bad
+    assert L.drx_gemm(drx.F32, 4, 6, 5, 1.0, None, 5, p, 6, 1.0, p, 6, None) == bad
+    # empty products return before any device work
+    assert L.drx_gemm(drx.F32, 0, 6, 5, 1.0, None, 5, None, 6, 1.0, None, 6, None) == 0
+    v = ctypes.c_int64()
+    assert L.drx_tune_set(b"gemm_tile", 3) == bad
+    assert L.drx_tune_set(b"gemm_tile", 2) == 0
+    assert L.drx_tune_get(b"gemm_tile", ctypes.byref(v)) == 0 and v.value == 2
+    assert L.drx_tune_set(b"gemm_tile", 0) == 0

@@ -414,3 +414,79 @@ def test_checks_catch_planted_mistakes(orc):
     zz = np.resize(A @ np.resize(w, shape[1]), shape[1])
     assert abs(np.dot(y, w) - np.dot(x, zz)) > 1e-6
     del wrong_t
+
+
+# ---------------------------------------------------------------------------
+# Dense tile product (sp gemm step): closed form, identities, numpy, linearity.
+# ---------------------------------------------------------------------------
+def _gemm_example_inputs(m, n, k):
+    # examples/sp/gemm_example.cpp:19-29 fills a[i][q] = (i + q) / m and
+    # b[q][j] = (q + j) / k.
+    i = np.arange(m)[:, None]
+    q = np.arange(k)
+    a = (i + q[None, :]) / m
+    b = (q[:, None] + np.arange(n)[None, :]) / k
+    return a, b
+
+
+def _gemm_example_closed_form(m, n, k):
+    # sum_q (i + q)(q + j) = k i j + (i + j) k(k-1)/2 + (k-1)k(2k-1)/6
+    i = np.arange(m, dtype=np.float64)[:, None]
+    j = np.arange(n, dtype=np.float64)[None, :]
+    s1 = k * (k - 1) / 2.0
+    s2 = (k - 1) * k * (2 * k - 1) / 6.0
+    return (k * i * j + (i + j) * s1 + s2) / (m * k)
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (7, 5, 3), (40, 33, 70)])
+def test_gemm_closed_form(orc, m, n, k):
+    a, b = _gemm_example_inputs(m, n, k)
+    out, absout = orc.gemm(a, b)
+    assert componentwise_err(out, _gemm_example_closed_form(m, n, k), absout) < 1e-14
+
+
+def test_gemm_identity_and_numpy(orc):
+    rng = np.random.default_rng(5)
+    for dt in (np.float32, np.float64):
+        a = rng.standard_normal((23, 17)).astype(dt)
+        b = rng.standard_normal((17, 29)).astype(dt)
+        assert np.array_equal(orc.gemm(a, np.eye(17, dtype=dt))[0], a.astype(np.float64))
+        assert np.array_equal(orc.gemm(np.eye(23, dtype=dt), a)[0], a.astype(np.float64))
+        out, absout = orc.gemm(a, b)
+        ref = a.astype(np.float64) @ b.astype(np.float64)
+        assert componentwise_err(out, ref, absout) < 1e-13
+
+
+def test_gemm_alpha_beta(orc):
+    rng = np.random.default_rng(6)
+    a = rng.standard_normal((9, 12))
+    b = rng.standard_normal((12, 10))
+    c = rng.standard_normal((9, 10))
+    base, absbase = orc.gemm(a, b)
+    out, _ = orc.gemm(a, b, c, alpha=-0.5, beta=2.0)
+    assert componentwise_err(out, -0.5 * base + 2.0 * c, np.abs(base) + np.abs(c) + absbase) < 1e-14
+    # beta == 0 does not read c (NaN there stays out of the result)
+    nan_c = np.full((9, 10), np.nan)
+    out0, _ = orc.gemm(a, b, nan_c, alpha=1.0, beta=0.0)
+    assert np.isfinite(out0).all()
+
+
+def test_gemm_checks_catch_planted_mistakes(orc):
+    m, n, k = 12, 9, 15
+    a, b = _gemm_example_inputs(m, n, k)
+    out, absout = orc.gemm(a, b)
+    exact = _gemm_example_closed_form(m, n, k)
+    # dropped term: the last k index lost
+    dropped = out - np.outer(a[:, -1], b[-1, :])
+    assert componentwise_err(dropped, exact, absout) > 1e-6
+    # off-by-one row of a
+    shifted = np.vstack([out[1:], out[-1:]])
+    assert componentwise_err(shifted, exact, absout) > 1e-6
+    # transposed operand: random (non-symmetric) inputs, b read as b^T
+    rng = np.random.default_rng(7)
+    ar = rng.standard_normal((10, 10))
+    br = rng.standard_normal((10, 10))
+    good, absr = orc.gemm(ar, br)
+    wrong = ar @ br.T
+    assert componentwise_err(good, ar @ br, absr) < 1e-13
+    assert componentwise_err(wrong, ar @ br, absr) > 1e-6
