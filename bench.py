@@ -36,6 +36,7 @@ sys.path.insert(0, os.path.join(REPO, "distributed-ranges_amd"))
 import drx  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFS = {"f32": 157.3, "f64": 78.6}  # dense matrix-core peaks of one MI355X
 SZ = {torch.float64: 8, torch.float32: 4, torch.bfloat16: 2, torch.float16: 2,
       torch.int32: 4, torch.int64: 8}
 
@@ -389,6 +390,40 @@ def form_gemveq_band(steps, cpu_seconds=10.0, with_cpu=True):
     return out
 
 
+def form_dense_gemm(steps, n=16000):
+    """The reference's dense benchmark shape (benchmarks/gbench/sp/gemm.cpp:
+    Gemm_DR, m = n = k = 16000, float): one tile product c += a b on the
+    matrix cores (drx_gemm), FP32 and FP64, beside torch.mm (hipBLASLt) on the
+    same operands. Inputs U[0, 100) as Gemm_Reference fills them; 16 rows of
+    the product are checked against float64 within (k + 2) u |a||b|."""
+    out = {"m": n, "n": n, "k": n, "flop_per_call": 2 * n ** 3}
+    peaks = {torch.float32: MFMA_PEAK_TFS["f32"], torch.float64: MFMA_PEAK_TFS["f64"]}
+    for dt, name in ((torch.float32, "f32"), (torch.float64, "f64")):
+        g = torch.Generator(device="cuda").manual_seed(3)
+        a = torch.rand(n, n, device="cuda", dtype=dt, generator=g) * 100
+        b = torch.rand(n, n, device="cuda", dtype=dt, generator=g) * 100
+        c = torch.zeros(n, n, device="cuda", dtype=dt)
+        reps = max(3, min(steps, 5))
+        _, ms = time_launches(lambda: drx.gemm(a, b, c), reps, 1)
+        _, ms_lib = time_launches(lambda: torch.mm(a, b, out=c), reps, 1)
+        c.zero_()
+        drx.gemm(a, b, c)
+        rows = torch.arange(0, n, n // 16, device="cuda")[:16]
+        ref = a[rows].double() @ b.double()
+        absref = a[rows].double().abs() @ b.double().abs()
+        u = 2.0 ** -24 if dt == torch.float32 else 2.0 ** -53
+        err = float(((c[rows].double() - ref).abs() / absref).max())
+        tfs = 2 * n ** 3 / ms / 1e9
+        out[name] = {"TFLOPs": round(tfs, 1), "frac": round(tfs / peaks[dt], 4),
+                     "peak_TFLOPs": peaks[dt], "ms_per_call": round(ms, 3),
+                     "torch_mm_TFLOPs": round(2 * n ** 3 / ms_lib / 1e9, 1),
+                     "vs_torch_mm": round(ms_lib / ms, 3),
+                     "rel_err_sample_rows": err, "bound_(k+2)u": (n + 2) * u}
+        del a, b, c, ref, absref
+        torch.cuda.empty_cache()
+    return out
+
+
 # ---------------------------------------------------------------------------
 def read_traffic():
     """HBM bytes per launch of the headline kernel from the committed
@@ -461,7 +496,8 @@ def main():
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(A, x, y, args.cpu_seconds)
     forms = {}
-    want = set(["laplacian2d_f64", "band_adjoint", "lowprec", "batched", "gemveq_band"]) \
+    want = set(["laplacian2d_f64", "band_adjoint", "lowprec", "batched", "gemveq_band",
+                "dense_gemm"]) \
         if args.forms == "all" else \
         set() if args.forms == "none" else set(args.forms.split(","))
     steps_f = max(5, min(args.steps, 20))
@@ -482,6 +518,9 @@ def main():
     if "gemveq_band" in want:
         log("form gemveq_band ...")
         forms["gemveq_band"] = form_gemveq_band(steps_f, args.cpu_seconds, not args.no_cpu)
+    if "dense_gemm" in want:
+        log("form dense_gemm ...")
+        forms["dense_gemm"] = form_dense_gemm(steps_f)
     if forms:
         result["forms"] = forms
     print(json.dumps(result), flush=True)

@@ -67,7 +67,8 @@ struct GemmCfg {
   static_assert(TM % 16 == 0 && TN % 16 == 0, "wave tile must be whole MFMA tiles");
   static_assert(A_VECS * V * kThreads == BM * BK, "A stage must split evenly");
   static_assert(B_VECS * V * kThreads == BK * BN, "B stage must split evenly");
-  static_assert(2 * STAGE * sizeof(T) <= 65536, "two stages must fit 64 KB of LDS");
+  static constexpr int ACC_REGS = FM * FN * 4 * static_cast<int>(sizeof(T)) / 4;
+  static_assert(2 * STAGE * sizeof(T) <= 160 * 1024, "two stages must fit 160 KB of LDS");
 };
 
 // Tile rows are taken GROUP_M at a time and swept column by column, so the
@@ -75,16 +76,18 @@ struct GemmCfg {
 // that XCD's L2.
 constexpr int kGroupM = 8;
 
-// Waves per SIMD the register budget is held to. FP32 keeps 64 accumulators
-// per lane: 4 waves with eight-wave workgroups, 3 with four-wave ones (their
-// lanes carry more of each stage's loads; 4 would spill). FP64 keeps 128: 2.
-template <typename T, int THREADS> constexpr int gemm_waves() {
-  return sizeof(T) == 4 ? (THREADS >= 512 ? 4 : 3) : 2;
+// Waves per SIMD the register budget is held to, from the accumulator
+// registers per lane: 64 -> 4 waves with eight-wave workgroups, 3 with
+// four-wave ones (their lanes carry more of each stage's loads; 4 would
+// spill); 128 -> 2; more -> 1.
+template <int ACC_REGS, int THREADS> constexpr int gemm_waves() {
+  return ACC_REGS <= 64 ? (THREADS >= 512 ? 4 : 3) : ACC_REGS <= 128 ? 2 : 1;
 }
 
 template <typename T, int BM, int BN, int BK, int WM, int WN>
 __global__ void __launch_bounds__(64 * WM * WN)
-    __attribute__((amdgpu_waves_per_eu(gemm_waves<T, 64 * WM * WN>())))
+    __attribute__((amdgpu_waves_per_eu(
+        gemm_waves<GemmCfg<T, BM, BN, BK, WM, WN>::ACC_REGS, 64 * WM * WN>())))
     k_gemm(int64_t m, int64_t n, int64_t K, T alpha, const T *__restrict__ A,
            int64_t lda, const T *__restrict__ B, int64_t ldb, T beta,
            T *__restrict__ C, int64_t ldc, int64_t tiles_m, int64_t tiles_n,
@@ -124,27 +127,31 @@ __global__ void __launch_bounds__(64 * WM * WN)
         const int kr = q / Cfg::B_TPR, jv = q % Cfg::B_TPR;
         rb[p] = *reinterpret_cast<const vec_t *>(B + (k0 + kr) * ldb + n0 + jv * Cfg::V);
       }
-    } else {  // edge tile, last partial stage, or unaligned operands
+    } else {  // edge tile, partial last stage or unaligned rows: per 16-byte slot
 #pragma unroll
       for (int p = 0; p < Cfg::A_VECS; ++p) {
         const int q = tid + p * Cfg::kThreads;
         const int row = q / Cfg::A_TPR, kv = q % Cfg::A_TPR;
-        const int64_t gi = m0 + row;
+        const int64_t gi = m0 + row, gk = k0 + kv * Cfg::V;
+        if (vec_ok && gi < m && gk + Cfg::V <= K) {
+          ra[p] = *reinterpret_cast<const vec_t *>(A + gi * lda + gk);
+        } else {
 #pragma unroll
-        for (int e = 0; e < Cfg::V; ++e) {
-          const int64_t gk = k0 + kv * Cfg::V + e;
-          ra[p][e] = (gi < m && gk < K) ? A[gi * lda + gk] : T(0);
+          for (int e = 0; e < Cfg::V; ++e)
+            ra[p][e] = (gi < m && gk + e < K) ? A[gi * lda + gk + e] : T(0);
         }
       }
 #pragma unroll
       for (int p = 0; p < Cfg::B_VECS; ++p) {
         const int q = tid + p * Cfg::kThreads;
         const int kr = q / Cfg::B_TPR, jv = q % Cfg::B_TPR;
-        const int64_t gk = k0 + kr;
+        const int64_t gk = k0 + kr, gj = n0 + jv * Cfg::V;
+        if (vec_ok && gk < K && gj + Cfg::V <= n) {
+          rb[p] = *reinterpret_cast<const vec_t *>(B + gk * ldb + gj);
+        } else {
 #pragma unroll
-        for (int e = 0; e < Cfg::V; ++e) {
-          const int64_t gj = n0 + jv * Cfg::V + e;
-          rb[p][e] = (gk < K && gj < n) ? B[gk * ldb + gj] : T(0);
+          for (int e = 0; e < Cfg::V; ++e)
+            rb[p][e] = (gk < K && gj + e < n) ? B[gk * ldb + gj + e] : T(0);
         }
       }
     }
@@ -244,22 +251,38 @@ void launch_gemm(int64_t m, int64_t n, int64_t k, T alpha, const T *a, int64_t l
   DRX_HIP(hipGetLastError());
 }
 
-// 256 x 128 tiles (eight waves) load fewer bytes per flop; they are taken when
-// there are enough of them to give every CU two workgroups.
-inline bool use_tall(int64_t m, int64_t n) {
-  const int t = tuned_gemm_tile();
-  if (t != 0) return t == 2;
-  return ((m + 255) / 256) * ((n + 127) / 128) >= 512;
+// Workgroup tiles (gemm_tile knob):
+//   1: 128 x 128, four waves of 64 x 64, BK = 16 (FP32) / 8 (FP64)
+//   2: 256 x 128, eight waves of 64 x 64, same depth
+//   3: 128 x 128, twice the depth          4: 256 x 128, twice the depth
+//   5: 256 x 256, eight waves of 64 x 128 (FP32 only; FP64 takes 4)
+// 0 picks by type and size (measured on 4096^3 .. 16384^3): FP64 takes 3;
+// FP32 takes 2 once C holds 2048 tiles of 256 x 128 (about 8192^2), 5 while
+// it holds at least 256 of 256 x 256 (about 4096^2), else 1.
+inline int default_gemm_tile(bool fp64, int64_t m, int64_t n) {
+  if (fp64) return 3;
+  if (((m + 255) / 256) * ((n + 127) / 128) >= 2048) return 2;
+  return ((m + 255) / 256) * ((n + 255) / 256) >= 256 ? 5 : 1;
 }
 
 template <typename T>
 void run_gemm(int64_t m, int64_t n, int64_t k, T alpha, const T *a, int64_t lda,
               const T *b, int64_t ldb, T beta, T *c, int64_t ldc, hipStream_t s) {
   constexpr int BK = sizeof(T) == 4 ? 16 : 8;
-  if (use_tall(m, n))
-    launch_gemm<T, 256, 128, BK, 4, 2>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s);
-  else
-    launch_gemm<T, 128, 128, BK, 2, 2>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s);
+  int t = tuned_gemm_tile();
+  if (t == 0) t = default_gemm_tile(sizeof(T) == 8, m, n);
+  switch (t) {
+    case 1: launch_gemm<T, 128, 128, BK, 2, 2>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s); break;
+    case 2: launch_gemm<T, 256, 128, BK, 4, 2>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s); break;
+    case 3: launch_gemm<T, 128, 128, 2 * BK, 2, 2>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s); break;
+    case 5:
+      if constexpr (sizeof(T) == 4) {
+        launch_gemm<T, 256, 256, BK, 4, 2>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s);
+        break;
+      }
+      [[fallthrough]];
+    default: launch_gemm<T, 256, 128, 2 * BK, 4, 2>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s); break;
+  }
 }
 
 }  // namespace

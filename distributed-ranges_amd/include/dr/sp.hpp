@@ -9,6 +9,12 @@
 //   gemv(c, a, b[, scratch])                    .../algorithms/matrix/gemv.hpp:78-124
 //   flat_gemv, gemv_square, gemv_square_copy    .../algorithms/matrix/gemv.hpp:19-76,126-207
 //   __detail::local_gemv (per tile, c += A b)   .../algorithms/matrix/local_gemv.hpp:83-98
+//   detail::factor, partition_matmul            .../containers/detail.hpp:15-24,
+//                                              .../containers/matrix_partition.hpp:96-117
+//   dense_matrix_view, distributed_dense_matrix .../views/dense_matrix_view.hpp,
+//                                              .../containers/distributed_dense_matrix.hpp:136-331
+//   gemm, gemm_inplace, gemm_buffered(_async)   .../algorithms/matrix/gemm.hpp:12-245
+//   __detail::local_gemm (per tile, c += a b)   .../algorithms/matrix/local_gemm.hpp:15-60
 //
 // A "device" of this model is a logical device: an entry of the list given
 // to init, each with its own HIP stream. Listing the same GPU several times
@@ -22,12 +28,14 @@
 
 #include <algorithm>
 #include <cassert>
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <iterator>
 #include <limits>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "dr/detail/drx.hpp"
@@ -105,10 +113,20 @@ public:
   virtual ~matrix_partition() = default;
 };
 
+namespace detail {
+// Factor pair (p, q) of n with q the largest divisor of n at most sqrt(n).
+inline dr::index<> factor(std::size_t n) {
+  std::size_t q = static_cast<std::size_t>(std::sqrt(static_cast<double>(n)));
+  while (q > 1 && n % q != 0) q--;
+  q = std::max<std::size_t>(q, 1);
+  return dr::index<>(n / q, q);
+}
+}  // namespace detail
+
 class block_cyclic final : public matrix_partition {
 public:
   block_cyclic(dr::index<> tile_shape = {tile::div, tile::div},
-               dr::index<> grid_shape = {nprocs(), 1})
+               dr::index<> grid_shape = detail::factor(nprocs()))
       : tile_shape_(tile_shape), grid_shape_(grid_shape) {}
 
   std::size_t tile_rank(dr::index<>, dr::index<> tile_id) const override {
@@ -575,6 +593,208 @@ void gemv_square_copy(distributed_vector<T> &c, sparse_matrix<T, I> &a,
         return dst;
       },
       true);
+}
+
+// ---------------------------------------------------------------------------
+// Dense matrices and their product
+// ---------------------------------------------------------------------------
+
+// The three partitions of c = a b over the processor grid factor(nprocs()):
+// c tiled by that grid, a and b sharing a k split sized to cover every rank.
+inline std::vector<block_cyclic> partition_matmul(std::size_t m, std::size_t n, std::size_t k) {
+  const dr::index<> c_grid = detail::factor(nprocs());
+  const std::size_t k_block = (m * k >= k * n) ? (nprocs() + c_grid[0] - 1) / c_grid[0]
+                                               : (nprocs() + c_grid[1] - 1) / c_grid[1];
+  return {block_cyclic({tile::div, tile::div}, {c_grid[0], k_block}),
+          block_cyclic({tile::div, tile::div}, {k_block, c_grid[1]}),
+          block_cyclic({tile::div, tile::div}, {c_grid[0], c_grid[1]})};
+}
+
+// A row-major window of device memory: shape, leading dimension, owner.
+template <typename T> class dense_matrix_view {
+public:
+  using key_type = dr::index<>;
+  dense_matrix_view(T *data, key_type shape, std::size_t ld, std::size_t rank)
+      : data_(data), shape_(shape), ld_(ld), rank_(rank) {}
+  T *data() const noexcept { return data_; }
+  key_type shape() const noexcept { return shape_; }
+  std::size_t size() const noexcept { return shape_[0] * shape_[1]; }
+  std::size_t ld() const noexcept { return ld_; }
+  std::size_t rank() const noexcept { return rank_; }
+
+private:
+  T *data_;
+  key_type shape_;
+  std::size_t ld_;
+  std::size_t rank_;
+};
+
+// Dense matrix cut into the partition's tiles, each a row-major block in HBM
+// placed on the tile's rank. Tiles start zeroed. Their rows are padded to a
+// multiple of 32 elements so that every row starts 16-byte aligned for the
+// matrix-core loads; ld() of a tile view reports the padded length.
+template <typename T> class distributed_dense_matrix {
+public:
+  using key_type = dr::index<>;
+  using size_type = std::size_t;
+
+  class element_ref {
+  public:
+    element_ref(T *p) : p_(p) {}
+    operator T() const {
+      T v;
+      dr::drx::hip_check(hipMemcpy(&v, p_, sizeof(T), hipMemcpyDeviceToHost), "read element");
+      return v;
+    }
+    element_ref &operator=(T v) {
+      dr::drx::hip_check(hipMemcpy(p_, &v, sizeof(T), hipMemcpyHostToDevice), "write element");
+      return *this;
+    }
+
+  private:
+    T *p_;
+  };
+
+  explicit distributed_dense_matrix(key_type shape)
+      : distributed_dense_matrix(shape, block_cyclic()) {}
+  distributed_dense_matrix(key_type shape, const matrix_partition &partition)
+      : shape_(shape), partition_(partition.clone()) {
+    grid_ = partition_->grid_shape(shape_);
+    tile_ = partition_->tile_shape(shape_);
+    ld_ = (tile_[1] + 31) / 32 * 32;
+    tiles_.reserve(grid_[0] * grid_[1]);
+    for (std::size_t i = 0; i < grid_[0]; i++)
+      for (std::size_t j = 0; j < grid_[1]; j++) {
+        tiles_.emplace_back(tile_[0] * ld_, partition_->tile_rank(shape_, key_type(i, j)));
+        if (tiles_.back().size())
+          dr::drx::hip_check(hipMemset(tiles_.back().data(), 0, tiles_.back().size() * sizeof(T)),
+                             "zero tile");
+      }
+  }
+
+  size_type size() const noexcept { return shape_[0] * shape_[1]; }
+  key_type shape() const noexcept { return shape_; }
+  key_type tile_shape() const noexcept { return tile_; }
+  key_type grid_shape() const noexcept { return grid_; }
+
+  dense_matrix_view<T> tile(key_type t) const {
+    const std::size_t tm = std::min(tile_[0], shape_[0] - t[0] * tile_[0]);
+    const std::size_t tn = std::min(tile_[1], shape_[1] - t[1] * tile_[1]);
+    const auto &tv = tiles_[t[0] * grid_[1] + t[1]];
+    return dense_matrix_view<T>(tv.data(), key_type(tm, tn), ld_, tv.rank());
+  }
+
+  std::vector<dense_matrix_view<T>> tiles() const {
+    std::vector<dense_matrix_view<T>> v;
+    for (std::size_t i = 0; i < grid_[0]; i++)
+      for (std::size_t j = 0; j < grid_[1]; j++) v.push_back(tile(key_type(i, j)));
+    return v;
+  }
+
+  // Element access through host copies (for setting up and checking).
+  element_ref operator[](key_type idx) {
+    const auto &tv = tiles_[idx[0] / tile_[0] * grid_[1] + idx[1] / tile_[1]];
+    return element_ref(tv.data() + (idx[0] % tile_[0]) * ld_ + idx[1] % tile_[1]);
+  }
+
+  // Whole-matrix copies from and to a dense row-major host array.
+  void copy_from_host(const T *host) {
+    for_tiles([&](dense_matrix_view<T> v, std::size_t r0, std::size_t c0) {
+      dr::drx::hip_check(hipMemcpy2D(v.data(), v.ld() * sizeof(T), host + r0 * shape_[1] + c0,
+                                     shape_[1] * sizeof(T), v.shape()[1] * sizeof(T),
+                                     v.shape()[0], hipMemcpyHostToDevice),
+                         "upload tile");
+    });
+  }
+  void copy_to_host(T *host) const {
+    for_tiles([&](dense_matrix_view<T> v, std::size_t r0, std::size_t c0) {
+      dr::drx::hip_check(hipMemcpy2D(host + r0 * shape_[1] + c0, shape_[1] * sizeof(T), v.data(),
+                                     v.ld() * sizeof(T), v.shape()[1] * sizeof(T), v.shape()[0],
+                                     hipMemcpyDeviceToHost),
+                         "download tile");
+    });
+  }
+
+private:
+  template <typename F> void for_tiles(F &&f) const {
+    for (std::size_t i = 0; i < grid_[0]; i++)
+      for (std::size_t j = 0; j < grid_[1]; j++) {
+        auto v = tile(key_type(i, j));
+        if (v.size()) f(v, i * tile_[0], j * tile_[1]);
+      }
+  }
+
+  key_type shape_, grid_, tile_;
+  std::size_t ld_ = 0;
+  std::unique_ptr<matrix_partition> partition_;
+  std::vector<device_vector<T>> tiles_;
+};
+
+namespace __detail {
+template <typename T> constexpr int dense_vtype() {
+  static_assert(std::is_same_v<T, float> || std::is_same_v<T, double>,
+                "the dense product takes float or double");
+  return std::is_same_v<T, float> ? DRX_F32 : DRX_F64;
+}
+
+// c += a b on one tile, asynchronously on q (drx_gemm, alpha = beta = 1).
+template <typename T>
+void local_gemm(hipStream_t q, dense_matrix_view<T> a, dense_matrix_view<T> b,
+                dense_matrix_view<T> c) {
+  assert(c.shape()[0] == a.shape()[0]);
+  assert(c.shape()[1] == b.shape()[1]);
+  assert(a.shape()[1] == b.shape()[0]);
+  dr::drx::check(drx_gemm(dense_vtype<T>(), c.shape()[0], c.shape()[1], a.shape()[1], 1.0,
+                          a.data(), std::max<std::size_t>(a.ld(), 1), b.data(),
+                          std::max<std::size_t>(b.ld(), 1), 1.0, c.data(),
+                          std::max<std::size_t>(c.ld(), 1), q),
+                 "drx_gemm");
+}
+}  // namespace __detail
+
+// c += a b. Each c tile's products run in order on its rank's stream, k
+// starting at (i + j) mod grid_k as the reference staggers them. The
+// reference's buffered forms copy remote a and b tiles into local buffers
+// first; on one GPU every tile is already in local HBM, so all four forms
+// issue the same tile products.
+template <typename T>
+void gemm_inplace(distributed_dense_matrix<T> &a, distributed_dense_matrix<T> &b,
+                  distributed_dense_matrix<T> &c) {
+  assert(c.shape()[0] == a.shape()[0]);
+  assert(c.shape()[1] == b.shape()[1]);
+  assert(a.shape()[1] == b.shape()[0]);
+  assert(c.grid_shape()[0] == a.grid_shape()[0]);
+  assert(c.grid_shape()[1] == b.grid_shape()[1]);
+  assert(a.grid_shape()[1] == b.grid_shape()[0]);
+  const std::size_t gk = a.grid_shape()[1];
+  for (std::size_t i = 0; i < c.grid_shape()[0]; i++)
+    for (std::size_t j = 0; j < c.grid_shape()[1]; j++) {
+      auto ct = c.tile(dr::index<>(i, j));
+      hipStream_t q = __detail::queue(ct.rank());
+      for (std::size_t k_ = 0; k_ < gk; k_++) {
+        const std::size_t k = (k_ + i + j) % gk;
+        __detail::local_gemm(q, a.tile(dr::index<>(i, k)), b.tile(dr::index<>(k, j)), ct);
+      }
+    }
+  __detail::wait_all();
+}
+
+template <typename T>
+void gemm_buffered(distributed_dense_matrix<T> &a, distributed_dense_matrix<T> &b,
+                   distributed_dense_matrix<T> &c) {
+  gemm_inplace(a, b, c);
+}
+
+template <typename T>
+void gemm_buffered_async(distributed_dense_matrix<T> &a, distributed_dense_matrix<T> &b,
+                         distributed_dense_matrix<T> &c) {
+  gemm_inplace(a, b, c);
+}
+
+template <typename T>
+void gemm(distributed_dense_matrix<T> &a, distributed_dense_matrix<T> &b,
+          distributed_dense_matrix<T> &c) {
+  gemm_buffered(a, b, c);
 }
 
 }  // namespace dr::sp

@@ -75,3 +75,9 @@ def test_mp_gemv_reference_tests():
 def test_sp_gemv_reference_tests():
     p = _run("test_sp_gemv")
     assert p.returncode == 0 and "0 failures" in p.stdout
+
+
+@pytest.mark.gpu
+def test_sp_gemm_example_checks():
+    p = _run("test_sp_gemm")
+    assert p.returncode == 0 and "0 failures" in p.stdout

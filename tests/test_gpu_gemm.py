@@ -34,7 +34,7 @@ def _bound(k, dtype):
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
-@pytest.mark.parametrize("tile", [1, 2])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("m,n,k", SHAPES)
 def test_gemm_integer_inputs_exact(orc, dtype, tile, m, n, k):
     rng = np.random.default_rng(m * 1000 + n * 10 + k)
@@ -143,9 +143,9 @@ def test_gemm_reference_example_inputs(dtype):
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_gemm_large_against_fp64_torch(dtype):
-    """4096^3 (512 tiles of 256 x 128, the default there) against torch's
-    float64 product of the same inputs, within the componentwise bound
-    (|A||B| from float64 too)."""
+    """4096^3 (FP32 on 256 x 256 tiles, FP64 on deep 128 x 128 ones: the
+    defaults there) against torch's float64 product of the same inputs,
+    within the componentwise bound (|A||B| from float64 too)."""
     n = 4096
     g = torch.Generator(device=DEV).manual_seed(1)
     a = torch.randn(n, n, device=DEV, dtype=dtype, generator=g)
