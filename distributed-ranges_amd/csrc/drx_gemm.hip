@@ -234,6 +234,203 @@ __global__ void __launch_bounds__(64 * WM * WN)
   }
 }
 
+// Direct-to-LDS variant for FP32, BK = 16, BN = 128 (A stage rows of 64
+// bytes, B stage rows of 512 bytes). global_load_lds_dwordx4 writes lane i of
+// a wave's instruction to 16 bytes at (wave's 1 KB block) + 16 i, so the LDS
+// image is unpadded and each lane picks the global slot that belongs at its
+// position. Slot c of a 16-row A block's row r sits at c ^ fa(r); slot c of B
+// stage row k at c ^ hb(k). Both swizzles keep the MFMA operand reads free of
+// bank conflicts (16 lanes per 256 bytes or 8 per 128 bytes for the 16-byte A
+// reads; 64 lanes per 256 bytes for the 4-byte B reads). Full stages load
+// this way, with no register round trip; rows past m read row m - 1 instead
+// (their products are never stored). Tiles on the n edge and a partial last
+// stage load through registers into the same positions, after the stage's
+// products, so that those registers are never live beside the operands.
+__device__ __forceinline__ int dl_fa(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
+// One 16-byte global -> LDS copy per lane into the wave's 1 KB block at lds.
+__device__ __forceinline__ void lds_dma16(const float *src, char *lds) {
+  __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)lds, 16, 0, 0);
+}
+__device__ __forceinline__ int dl_hb(int k) { return 4 * ((k >> 2) & 3); }
+
+// Register budget: 3 waves per SIMD (the kernel needs about 136 VGPRs; at
+// 128 it spills).
+template <int BM, int WM, int WN>
+__global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(3)))
+    k_gemm_dl(int64_t m, int64_t n, int64_t K, float alpha, const float *__restrict__ A,
+              int64_t lda, const float *__restrict__ B, int64_t ldb, float beta,
+              float *__restrict__ C, int64_t ldc, int64_t tiles_m, int64_t tiles_n,
+              int64_t per_xcd, int vec_ok) {
+  constexpr int BN = 128, BK = 16, NW = WM * WN;
+  constexpr int A_BYTES = BM * BK * 4, B_BYTES = BK * BN * 4, STAGE = A_BYTES + B_BYTES;
+  constexpr int A_INSTR = A_BYTES / 1024 / NW, B_INSTR = B_BYTES / 1024 / NW;
+  constexpr int TM = BM / WM, TN = BN / WN, FM = TM / 16, FN = TN / 16;
+  static_assert(A_INSTR * NW * 1024 == A_BYTES && B_INSTR * NW * 1024 == B_BYTES,
+                "stages must split into whole 1 KB wave blocks");
+  static_assert(TM % 16 == 0 && TN % 16 == 0, "wave tile must be whole MFMA tiles");
+  __shared__ __attribute__((aligned(1024))) char smem[2 * STAGE];
+
+  const int64_t logical = map_block(blockIdx.x, per_xcd);
+  if (logical >= tiles_m * tiles_n) return;  // whole workgroup leaves together
+  const int64_t per_group = int64_t(kGroupM) * tiles_n;
+  const int64_t first = logical / per_group * kGroupM;
+  const int64_t gsz = tiles_m - first < kGroupM ? tiles_m - first : kGroupM;
+  const int64_t in_g = logical % per_group;
+  const int64_t m0 = (first + in_g % gsz) * BM;
+  const int64_t n0 = (in_g / gsz) * BN;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int wm = wave / WN, wn = wave % WN;
+  const bool n_inner = vec_ok && n0 + BN <= n;
+
+  // The (row, slot) each lane moves: A block p * NW + wave, row block*16 +
+  // lane/4, slot (lane%4) ^ fa(lane/4); B block p * NW + wave, row
+  // 2 block + lane/32, slot (lane%32) ^ hb(row).
+  auto a_row = [&](int p) { return (p * NW + wave) * 16 + (lane >> 2); };
+  auto a_col = [&](int) { return 4 * ((lane & 3) ^ dl_fa(lane >> 2)); };
+  auto b_row = [&](int p) { return (p * NW + wave) * 2 + (lane >> 5); };
+  auto b_col = [&](int p) { return 4 * ((lane & 31) ^ dl_hb(b_row(p))); };
+  const float *a_src[A_INSTR];
+  const float *b_src[B_INSTR];
+#pragma unroll
+  for (int p = 0; p < A_INSTR; ++p) {
+    const int64_t gi = m0 + a_row(p) < m ? m0 + a_row(p) : m - 1;
+    a_src[p] = A + gi * lda + a_col(p);
+  }
+#pragma unroll
+  for (int p = 0; p < B_INSTR; ++p) b_src[p] = B + b_row(p) * ldb + n0 + b_col(p);
+
+  auto issue_direct = [&](int64_t k0, char *st) {
+#pragma unroll
+    for (int p = 0; p < A_INSTR; ++p)
+      lds_dma16(a_src[p] + k0, st + (p * NW + wave) * 1024);
+#pragma unroll
+    for (int p = 0; p < B_INSTR; ++p)
+      lds_dma16(b_src[p] + k0 * ldb, st + A_BYTES + (p * NW + wave) * 1024);
+  };
+
+  f32x4 ra[A_INSTR], rb[B_INSTR];
+  auto load_regs = [&](int64_t k0) {
+#pragma unroll
+    for (int p = 0; p < A_INSTR; ++p) {
+      const int64_t gi = m0 + a_row(p), gk = k0 + a_col(p);
+      if (vec_ok && gi < m && gk + 4 <= K) {
+        ra[p] = *reinterpret_cast<const f32x4 *>(A + gi * lda + gk);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) ra[p][e] = (gi < m && gk + e < K) ? A[gi * lda + gk + e] : 0.f;
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < B_INSTR; ++p) {
+      const int64_t gk = k0 + b_row(p), gj = n0 + b_col(p);
+      if (vec_ok && gk < K && gj + 4 <= n) {
+        rb[p] = *reinterpret_cast<const f32x4 *>(B + gk * ldb + gj);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) rb[p][e] = (gk < K && gj + e < n) ? B[gk * ldb + gj + e] : 0.f;
+      }
+    }
+  };
+  auto store_regs = [&](char *st) {
+#pragma unroll
+    for (int p = 0; p < A_INSTR; ++p)
+      *reinterpret_cast<f32x4 *>(st + (p * NW + wave) * 1024 + lane * 16) = ra[p];
+#pragma unroll
+    for (int p = 0; p < B_INSTR; ++p)
+      *reinterpret_cast<f32x4 *>(st + A_BYTES + (p * NW + wave) * 1024 + lane * 16) = rb[p];
+  };
+
+  // Operand offsets (bytes within a stage) of this lane. Row block i of the
+  // wave's A tile is a_off + 1024 i; B row 4 g + kk of column tile j is
+  // b_off[j] + 512 kk (hb is the same for the four rows of lane group g).
+  const int a_off = wm * TM / 16 * 1024 + r * 64 + (g ^ dl_fa(r)) * 16;
+  int b_off[FN];
+#pragma unroll
+  for (int j = 0; j < FN; ++j) {
+    const int col = wn * TN + j * 16 + r;
+    b_off[j] = A_BYTES + 4 * g * 512 + (((col >> 2) ^ dl_hb(4 * g)) * 16) + (col & 3) * 4;
+  }
+
+  f32x4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = f32x4{};
+
+  const int64_t nk = (K + BK - 1) / BK;
+  if (nk > 0) {
+    if (n_inner && BK <= K) {
+      issue_direct(0, smem);
+    } else {
+      load_regs(0);
+      store_regs(smem);
+    }
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    const char *cur = smem + (kt & 1) * STAGE;
+    char *nxt = smem + ((kt + 1) & 1) * STAGE;
+    const bool more = kt + 1 < nk;
+    const bool direct = more && n_inner && (kt + 2) * BK <= K;
+    if (direct) issue_direct((kt + 1) * BK, nxt);
+    f32x4 a[FM];
+#pragma unroll
+    for (int i = 0; i < FM; ++i) a[i] = *reinterpret_cast<const f32x4 *>(cur + a_off + i * 1024);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      float b[FN];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) b[j] = *reinterpret_cast<const float *>(cur + b_off[j] + kk * 512);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<float>::op(a[i][kk], b[j], acc[i][j]);
+    }
+    if (more && !direct) {
+      load_regs((kt + 1) * BK);
+      store_regs(nxt);
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t gi = m0 + wm * TM + i * 16 + Mfma<float>::row(g, v);
+      if (gi >= m) continue;
+      float *crow = C + gi * ldc;
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int64_t gj = n0 + wn * TN + j * 16 + r;
+        if (gj >= n) continue;
+        const float prod = alpha * acc[i][j][v];
+        crow[gj] = beta == 0.f ? prod : prod + beta * crow[gj];
+      }
+    }
+  }
+}
+
+template <int BM, int WM, int WN>
+void launch_gemm_dl(int64_t m, int64_t n, int64_t k, float alpha, const float *a, int64_t lda,
+                    const float *b, int64_t ldb, float beta, float *c, int64_t ldc,
+                    hipStream_t s) {
+  const int64_t tiles_m = (m + BM - 1) / BM, tiles_n = (n + 127) / 128;
+  const int64_t per_xcd = (tiles_m * tiles_n + kXcds - 1) / kXcds;
+  DRX_REQUIRE(per_xcd * kXcds <= int64_t(0xffffffff), "product too large for one launch");
+  const bool aligned = reinterpret_cast<uintptr_t>(a) % 16 == 0 &&
+                       reinterpret_cast<uintptr_t>(b) % 16 == 0 && lda % 4 == 0 && ldb % 4 == 0;
+  hipLaunchKernelGGL((k_gemm_dl<BM, WM, WN>), dim3(static_cast<unsigned>(per_xcd * kXcds)),
+                     dim3(64 * WM * WN), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                     tiles_m, tiles_n, per_xcd, aligned ? 1 : 0);
+  DRX_HIP(hipGetLastError());
+}
+
 template <typename T, int BM, int BN, int BK, int WM, int WN>
 void launch_gemm(int64_t m, int64_t n, int64_t k, T alpha, const T *a, int64_t lda,
                  const T *b, int64_t ldb, T beta, T *c, int64_t ldc, hipStream_t s) {
@@ -256,12 +453,14 @@ void launch_gemm(int64_t m, int64_t n, int64_t k, T alpha, const T *a, int64_t l
 //   2: 256 x 128, eight waves of 64 x 64, same depth
 //   3: 128 x 128, twice the depth          4: 256 x 128, twice the depth
 //   5: 256 x 256, eight waves of 64 x 128 (FP32 only; FP64 takes 4)
+//   6, 7: 128 x 128 and 256 x 128 loading stages straight into LDS (FP32
+//      only; FP64 takes 4)
 // 0 picks by type and size (measured on 4096^3 .. 16384^3): FP64 takes 3;
-// FP32 takes 2 once C holds 2048 tiles of 256 x 128 (about 8192^2), 5 while
+// FP32 takes 6 once C holds 4096 tiles of 128 x 128 (about 8192^2), 5 while
 // it holds at least 256 of 256 x 256 (about 4096^2), else 1.
 inline int default_gemm_tile(bool fp64, int64_t m, int64_t n) {
   if (fp64) return 3;
-  if (((m + 255) / 256) * ((n + 127) / 128) >= 2048) return 2;
+  if (((m + 127) / 128) * ((n + 127) / 128) >= 4096) return 6;
   return ((m + 255) / 256) * ((n + 255) / 256) >= 256 ? 5 : 1;
 }
 
@@ -275,6 +474,17 @@ void run_gemm(int64_t m, int64_t n, int64_t k, T alpha, const T *a, int64_t lda,
     case 1: launch_gemm<T, 128, 128, BK, 2, 2>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s); break;
     case 2: launch_gemm<T, 256, 128, BK, 4, 2>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s); break;
     case 3: launch_gemm<T, 128, 128, 2 * BK, 2, 2>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s); break;
+    case 6:
+    case 7:
+      if constexpr (sizeof(T) == 4) {
+        if (t == 6)
+          launch_gemm_dl<128, 2, 2>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s);
+        else
+          launch_gemm_dl<256, 4, 2>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s);
+        break;
+      }
+      launch_gemm<T, 256, 128, 2 * BK, 4, 2>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s);
+      break;
     case 5:
       if constexpr (sizeof(T) == 4) {
         launch_gemm<T, 256, 256, BK, 4, 2>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s);

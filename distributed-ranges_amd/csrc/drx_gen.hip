@@ -356,7 +356,7 @@ int drx_tune_set(const char *key, int64_t value) {
       DRX_REQUIRE(value >= 0 && value <= (int64_t(1) << 30), "grid_cap out of range");
       g_grid_cap.store(value);
     } else if (std::strcmp(key, "gemm_tile") == 0) {
-      DRX_REQUIRE(value >= 0 && value <= 5, "gemm_tile must be in [0, 5]");
+      DRX_REQUIRE(value >= 0 && value <= 7, "gemm_tile must be in [0, 7]");
       g_gemm_tile.store(static_cast<int>(value));
     } else {
       throw Error(DRX_ERR_INVALID, std::string("unknown tuning key: ") + key);

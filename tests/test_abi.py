@@ -89,7 +89,7 @@ def test_gemm_argument_validation_without_gpu():
     # empty products return before any device work
     assert L.drx_gemm(drx.F32, 0, 6, 5, 1.0, None, 5, None, 6, 1.0, None, 6, None) == 0
     v = ctypes.c_int64()
-    assert L.drx_tune_set(b"gemm_tile", 6) == bad
+    assert L.drx_tune_set(b"gemm_tile", 8) == bad
     assert L.drx_tune_set(b"gemm_tile", 2) == 0
     assert L.drx_tune_get(b"gemm_tile", ctypes.byref(v)) == 0 and v.value == 2
     assert L.drx_tune_set(b"gemm_tile", 0) == 0

@@ -34,7 +34,7 @@ def _bound(k, dtype):
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
-@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6, 7])
 @pytest.mark.parametrize("m,n,k", SHAPES)
 def test_gemm_integer_inputs_exact(orc, dtype, tile, m, n, k):
     rng = np.random.default_rng(m * 1000 + n * 10 + k)
