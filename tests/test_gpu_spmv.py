@@ -350,6 +350,39 @@ def test_low_precision_storage(orc, vdt, xdt_same):
     assert componentwise_err(y.cpu().numpy(), y64, abs64) < (2 if xdt_same else 1) * unit + 1e-5
 
 
+@pytest.mark.parametrize("vdtype", [torch.float64, torch.float32])
+def test_share_beyond_2_31_entries(vdtype):
+    """A share of 2^24 rows and 2.16e9 > 2^31 entries (band +-64, int64
+    indices; values 1): every output row equals its closed form exactly.
+    x_i = i (FP64) or i mod 1024 (FP32, every partial sum an integer below
+    2^24), so rounding cannot hide an error. The CSC build and the panel
+    layout (radix sort, 32-bit segment ids) refuse such a share with a clear
+    error instead of running."""
+    n, hw = 1 << 24, 64
+    A = drx.gen.band(n, hw, hw, vdtype=vdtype, itype=torch.int64, seed=5)
+    assert A.nnz > 2 ** 31
+    A.vals.fill_(1.0)
+    i = torch.arange(n, device=DEV, dtype=torch.int64)
+    xi = i if vdtype == torch.float64 else i % 1024
+    y = A.matvec(xi.to(vdtype))
+    lo = torch.clamp(i - hw, min=0)
+    hi = torch.clamp(i + hw, max=n - 1)
+    if vdtype == torch.float64:
+        exact = (lo + hi) * (hi - lo + 1) // 2
+    else:
+        cs = torch.cumsum(xi, 0)
+        exact = cs[hi] - torch.where(lo > 0, cs[(lo - 1).clamp_min(0)], torch.zeros_like(lo))
+    torch.cuda.synchronize()
+    assert torch.equal(y.to(torch.int64), exact)
+    with pytest.raises(drx.DrxError, match="2\\^31"):
+        A.transpose()
+    with pytest.raises(drx.DrxError, match="2\\^31"):
+        A.set_panels()
+    A.close()
+    del A, y, exact, i, xi
+    torch.cuda.empty_cache()
+
+
 def test_zipf_config2_full_size_properties(orc):
     """16M x 16M, Zipf(1.5) rows clipped to [1, 4096], FP32: 3050 sampled
     rows (incl. the 50 longest) against the oracle, and exact
