@@ -147,6 +147,34 @@ template <int G, typename T> __device__ __forceinline__ T group_sum(T v) {
   return v;
 }
 
+// Wave-wide minimum and maximum of a 32-bit value: DPP inside 16-lane rows,
+// then the four rows' results through readlane (scalar, wave-uniform).
+template <int CTRL> __device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+  return static_cast<uint32_t>(
+      __builtin_amdgcn_mov_dpp(static_cast<int>(v), CTRL, 0xf, 0xf, false));
+}
+__device__ __forceinline__ void wave_min_max(uint32_t &lo, uint32_t &hi) {
+  uint32_t t;
+  t = dpp_u32<kQuadXor1>(lo); lo = t < lo ? t : lo;
+  t = dpp_u32<kQuadXor1>(hi); hi = t > hi ? t : hi;
+  t = dpp_u32<kQuadXor2>(lo); lo = t < lo ? t : lo;
+  t = dpp_u32<kQuadXor2>(hi); hi = t > hi ? t : hi;
+  t = dpp_u32<kRowHalfMirror>(lo); lo = t < lo ? t : lo;
+  t = dpp_u32<kRowHalfMirror>(hi); hi = t > hi ? t : hi;
+  t = dpp_u32<kRowMirror>(lo); lo = t < lo ? t : lo;
+  t = dpp_u32<kRowMirror>(hi); hi = t > hi ? t : hi;
+  uint32_t l = 0xffffffffu, h = 0;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const uint32_t lr = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(lo), 16 * r));
+    const uint32_t hr = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(hi), 16 * r));
+    l = lr < l ? lr : l;
+    h = hr > h ? hr : h;
+  }
+  lo = l;
+  hi = h;
+}
+
 // ---------------------------------------------------------------------------
 // Row spans of a CSR share. rowptr holds absolute offsets; base is the
 // share's first nnz; the row's span is clamped into [0, nnz). Row `nrows` is

@@ -45,6 +45,7 @@ struct FwdArgs {
   int64_t ldy, k;
   double alpha, beta;
   int gshift;  // lane-group choice uses (mean row length >> gshift)
+  int xstage;  // kPlain, k = 1: stage each block's x window in LDS
   const void *xr;  // k >= 2: x in row-major chunks of KP columns
   int64_t ncols;   // columns of the share (rows of one xr chunk)
 };
@@ -150,6 +151,7 @@ template <int WG, int CAP, bool ROWMAP, typename TA, typename TI> struct FwdSmem
   TA red[WG / 64];
   TI map_s[MAPCAP];       // kRowMap: output rows of the block
   int rp_s[ROWCAP + 1];
+  uint32_t xmm[2 * (WG / 64)];  // per-wave column min / max (x staging)
 };
 
 // Storage value from its raw bits (2- or 4-byte types).
@@ -171,11 +173,13 @@ template <typename TV, typename U> __device__ __forceinline__ TV bits_to(U u) {
 // arrays' last entry) inside that same aligned pair; such entries are read
 // but never used.
 template <int WG, int CAP, int MODE, bool BUFX, typename TV, typename TX,
-          typename TY, typename TA, typename TI, int VW = 1>
+          typename TY, typename TA, typename TI, int VW = 1, bool XS = false>
 __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
                                               FwdSmem<WG, CAP, MODE == kRowMap, TA, TI> &sm) {
   static_assert(VW == 1 || (VW == 2 && MODE != kPerm && sizeof(TI) == 4 && sizeof(TV) <= 4),
                 "pairs of entries: plain or row-map mode, 32-bit indices, <= 4-byte values");
+  static_assert(!XS || (MODE == kPlain && BUFX && VW == 1),
+                "x staging: plain mode, buffer gathers, one entry per load");
   constexpr int T = CAP / WG;
   constexpr int ROWCAP = CAP / 2;
   TA *prod = sm.prod;
@@ -276,9 +280,65 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
       }
     }
 
+    // x staged in LDS (plain CSR, one right-hand side, columns local to the
+    // block): the window's columns span [lo, hi]; x[lo, lo + xn) is loaded
+    // once, coalesced, into the product buffer (xn <= its capacity), and each
+    // entry reads its x from there -- or from memory past xn. The products
+    // equal the gathered ones bit for bit.
+    bool staged = false;
+    if constexpr (XS) {
+      if (a.k == 1) {
+        constexpr uint32_t XCAP = CAP * sizeof(TA) / sizeof(TX);
+        uint32_t lo = 0xffffffffu, hi = 0;
+#pragma unroll
+        for (int u = 0; u < T; u++) {
+          if (tid + u * WG < n_s) {
+            lo = c[u] < lo ? c[u] : lo;
+            hi = c[u] > hi ? c[u] : hi;
+          }
+        }
+        wave_min_max(lo, hi);
+        if ((tid & 63) == 0) {
+          sm.xmm[tid >> 6] = lo;
+          sm.xmm[WG / 64 + (tid >> 6)] = hi;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < WG / 64; w++) {
+          lo = sm.xmm[w] < lo ? sm.xmm[w] : lo;
+          hi = sm.xmm[WG / 64 + w] > hi ? sm.xmm[WG / 64 + w] : hi;
+        }
+        if (lo <= hi) {
+          const uint32_t xn = hi - lo + 1 < XCAP ? hi - lo + 1 : XCAP;
+          TX *xs = reinterpret_cast<TX *>(prod);
+          const rsrc_t rx = make_rsrc(x, a.xbytes);
+          for (uint32_t i = tid; i < xn; i += WG) xs[i] = buf_load<TX>(rx, (lo + i) * sizeof(TX));
+          __syncthreads();
+#pragma unroll
+          for (int u = 0; u < T; u++) {
+            if (tid + u * WG < n_s) {
+              const uint32_t off = c[u] - lo;
+              TA xv;
+              if (off < xn)
+                xv = to_acc<TA>(xs[off]);
+              else
+                xv = to_acc<TA>(buf_load<TX>(rx, c[u] * sizeof(TX)));
+              v[u] = v[u] * xv;
+            }
+          }
+          __syncthreads();
+#pragma unroll
+          for (int u = 0; u < T; u++)
+            if (tid + u * WG < n_s) prod[tid + u * WG] = v[u];
+          staged = true;
+        }
+      }
+    }
+
     for (int64_t j = 0; j < a.k; j++) {
       const TX *xj = x + j * a.ldx;
-      if constexpr (BUFX) {
+      if (staged) {
+      } else if constexpr (BUFX) {
         const rsrc_t rx = make_rsrc(xj, a.xbytes);
 #pragma unroll
         for (int u = 0; u < T; u++) {
@@ -359,7 +419,7 @@ template <int WG, int CAP, int MODE, typename TA, typename TI> constexpr int fwd
 }
 
 template <int WG, int CAP, int MODE, bool BUFX, typename TV, typename TX,
-          typename TY, typename TA, typename TI, int VW = 1>
+          typename TY, typename TA, typename TI, int VW = 1, bool XS = false>
 __global__ __launch_bounds__(WG)
 __attribute__((amdgpu_waves_per_eu(fwd_waves<WG, CAP, MODE, TA, TI>(), 8)))
 void k_spmv_fwd(FwdArgs a) {
@@ -367,7 +427,7 @@ void k_spmv_fwd(FwdArgs a) {
   for (int64_t wi = blockIdx.x; wi < a.per_xcd * kXcds; wi += a.grid) {
     const int64_t blk = map_block(wi, a.per_xcd);
     if (blk >= a.nblocks) continue;
-    process_block<WG, CAP, MODE, BUFX, TV, TX, TY, TA, TI, VW>(a, blk, sm);
+    process_block<WG, CAP, MODE, BUFX, TV, TX, TY, TA, TI, VW, XS>(a, blk, sm);
   }
 }
 

@@ -34,6 +34,8 @@ int tuned_vec() { return g_vec.load(); }
 int64_t tuned_grid_cap() { return g_grid_cap.load(); }
 static std::atomic<int> g_gemm_tile{0};
 int tuned_gemm_tile() { return g_gemm_tile.load(); }
+static std::atomic<int> g_xstage{1};
+int tuned_xstage() { return g_xstage.load(); }
 
 int64_t tuned_block_nnz() {
   int64_t v = g_block_nnz.load();
@@ -355,6 +357,9 @@ int drx_tune_set(const char *key, int64_t value) {
     } else if (std::strcmp(key, "grid_cap") == 0) {
       DRX_REQUIRE(value >= 0 && value <= (int64_t(1) << 30), "grid_cap out of range");
       g_grid_cap.store(value);
+    } else if (std::strcmp(key, "xstage") == 0) {
+      DRX_REQUIRE(value >= 0 && value <= 2, "xstage must be 0, 1 or 2");
+      g_xstage.store(static_cast<int>(value));
     } else if (std::strcmp(key, "gemm_tile") == 0) {
       DRX_REQUIRE(value >= 0 && value <= 7, "gemm_tile must be in [0, 7]");
       g_gemm_tile.store(static_cast<int>(value));
@@ -383,6 +388,8 @@ int drx_tune_get(const char *key, int64_t *value) {
       *value = g_grid_cap.load();
     else if (std::strcmp(key, "gemm_tile") == 0)
       *value = g_gemm_tile.load();
+    else if (std::strcmp(key, "xstage") == 0)
+      *value = g_xstage.load();
     else
       throw Error(DRX_ERR_INVALID, std::string("unknown tuning key: ") + key);
   });

@@ -93,3 +93,6 @@ def test_gemm_argument_validation_without_gpu():
     assert L.drx_tune_set(b"gemm_tile", 2) == 0
     assert L.drx_tune_get(b"gemm_tile", ctypes.byref(v)) == 0 and v.value == 2
     assert L.drx_tune_set(b"gemm_tile", 0) == 0
+    assert L.drx_tune_get(b"xstage", ctypes.byref(v)) == 0 and v.value == 1
+    assert L.drx_tune_set(b"xstage", 3) == bad
+    assert L.drx_tune_set(b"xstage", 2) == 0 and L.drx_tune_set(b"xstage", 1) == 0

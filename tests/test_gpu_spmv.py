@@ -602,6 +602,60 @@ def test_paired_entry_loads_bitexact(orc, vdt, block_nnz):
         assert componentwise_err(p2.cpu().numpy(), ys, absy) < 1e-5
 
 
+def _staging_cases(rng):
+    """(name, rowptr, colind, vals, shape): a band (every block's columns fit
+    its LDS window), a 2-D 5-point stencil (windows a little wider than the
+    FP64 LDS capacity: some entries read x from memory), random columns
+    (spans far wider), empty rows, rows longer than a window, one column."""
+    out = []
+    n = 30000
+    A = sp.diags([rng.uniform(-1, 1, n - abs(o)) for o in range(-16, 17)], list(range(-16, 17)),
+                 format="csr")
+    out.append(("band", A))
+    g = 300
+    T = sp.diags([4.0 * np.ones(g), -np.ones(g - 1), -np.ones(g - 1)], [0, -1, 1])
+    I = sp.identity(g)
+    L = (sp.kron(I, T) + sp.kron(sp.diags([-np.ones(g - 1), -np.ones(g - 1)], [-1, 1]), I)).tocsr()
+    out.append(("lap2d", L))
+    out.append(("random", np_rand_csr(5000, 40000, 0.002, 7, empty_rows=True)))
+    lens = [0, 5000, 3, 0, 9000, 1, 2, 700] + list(rng.integers(0, 30, 2000))
+    m = len(lens)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    ci = np.concatenate([np.sort(rng.choice(12000, L, replace=False)) for L in lens]).astype(np.int64)
+    out.append(("long_rows", sp.csr_matrix((rng.uniform(-1, 1, ci.size), ci, rp), shape=(m, 12000))))
+    C = sp.csr_matrix((rng.uniform(-1, 1, 4000), np.zeros(4000, np.int64), np.arange(4001)),
+                      shape=(4000, 1))
+    out.append(("one_column", C))
+    return out
+
+
+@pytest.mark.parametrize("vdt", [torch.float64, torch.float32])
+@pytest.mark.parametrize("itype", [torch.int32, torch.int64])
+def test_x_staging_bitexact(orc, vdt, itype):
+    """xstage = 2 (every block stages its x window in LDS) gives the same
+    bits as per-entry gathers (xstage = 0), and xstage = 1 (the default:
+    only over locally clustered columns) as well; all within the oracle
+    tolerance."""
+    rng = np.random.default_rng(17)
+    for name, A in _staging_cases(rng):
+        A = A.tocsr()
+        A.sort_indices()
+        rp, ci = A.indptr.astype(np.int64), A.indices.astype(np.int64)
+        v = A.data.astype(np.float32 if vdt == torch.float32 else np.float64)
+        Ad = upload(rp, ci, v, A.shape, itype=itype)
+        x = drx.gen.uniform(A.shape[1], vdt, -1, 1, seed=11)
+        ys = {}
+        for xs in (0, 2, 1):
+            with drx.tuned(xstage=xs):
+                ys[xs] = Ad.matvec(x)
+        torch.cuda.synchronize()
+        assert torch.equal(ys[0], ys[2]), name
+        assert torch.equal(ys[0], ys[1]), name
+        yo, absy = orc.spmv(rp, ci, v, x.cpu().numpy())
+        assert componentwise_err(ys[2].cpu().numpy(), yo, absy) < TOL[vdt], name
+        Ad.close()
+
+
 @pytest.mark.parametrize("cap", [8, 64, 1000])
 def test_grid_cap_loops_over_blocks(orc, cap):
     """A capped grid (workgroups loop over the row blocks, keeping their XCD)
