@@ -175,6 +175,36 @@ __device__ __forceinline__ void wave_min_max(uint32_t &lo, uint32_t &hi) {
   hi = h;
 }
 
+// Smallest and largest valid column index of a workgroup's window: lane
+// entries tid + u * WG < n (c[u] for u < T). xmm holds 2 * WG / 64 words of
+// LDS. Ends after a barrier with the same lo, hi in every lane (lo > hi when
+// the window is empty).
+template <int WG, int T>
+__device__ __forceinline__ void block_col_range(const uint32_t (&c)[T], int n, uint32_t *xmm,
+                                                uint32_t &lo, uint32_t &hi) {
+  const int tid = threadIdx.x;
+  lo = 0xffffffffu;
+  hi = 0;
+#pragma unroll
+  for (int u = 0; u < T; u++) {
+    if (tid + u * WG < n) {
+      lo = c[u] < lo ? c[u] : lo;
+      hi = c[u] > hi ? c[u] : hi;
+    }
+  }
+  wave_min_max(lo, hi);
+  if ((tid & 63) == 0) {
+    xmm[tid >> 6] = lo;
+    xmm[WG / 64 + (tid >> 6)] = hi;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < WG / 64; w++) {
+    lo = xmm[w] < lo ? xmm[w] : lo;
+    hi = xmm[WG / 64 + w] > hi ? xmm[WG / 64 + w] : hi;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Row spans of a CSR share. rowptr holds absolute offsets; base is the
 // share's first nnz; the row's span is clamped into [0, nnz). Row `nrows` is

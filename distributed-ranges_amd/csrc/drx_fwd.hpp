@@ -289,25 +289,8 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
     if constexpr (XS) {
       if (a.k == 1) {
         constexpr uint32_t XCAP = CAP * sizeof(TA) / sizeof(TX);
-        uint32_t lo = 0xffffffffu, hi = 0;
-#pragma unroll
-        for (int u = 0; u < T; u++) {
-          if (tid + u * WG < n_s) {
-            lo = c[u] < lo ? c[u] : lo;
-            hi = c[u] > hi ? c[u] : hi;
-          }
-        }
-        wave_min_max(lo, hi);
-        if ((tid & 63) == 0) {
-          sm.xmm[tid >> 6] = lo;
-          sm.xmm[WG / 64 + (tid >> 6)] = hi;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int w = 0; w < WG / 64; w++) {
-          lo = sm.xmm[w] < lo ? sm.xmm[w] : lo;
-          hi = sm.xmm[WG / 64 + w] > hi ? sm.xmm[WG / 64 + w] : hi;
-        }
+        uint32_t lo, hi;
+        block_col_range<WG, T>(c, n_s, sm.xmm, lo, hi);
         if (lo <= hi) {
           const uint32_t xn = hi - lo + 1 < XCAP ? hi - lo + 1 : XCAP;
           TX *xs = reinterpret_cast<TX *>(prod);

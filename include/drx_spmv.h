@@ -92,14 +92,15 @@ const char *drx_last_error(void);
  *                 is the default 1024; 0: the plan's own blocks.
  *   "grid_cap"    most workgroups of a per-share launch (0 = one per row
  *                 block); capped launches loop over the blocks.
- *   "xstage"      one right-hand side, plain CSR: 1 (default) each row
- *                 block loads the x window its columns span into LDS once
- *                 and reads x from there, for FP32 shares whose blocks span
- *                 at most half as many columns as entries (bands; probed
- *                 once per plan and colind, with one wait on the stream);
- *                 0 never; 2 always (any dtype; a window wider than the
- *                 block's LDS buffer reads the rest from memory). Results
- *                 are the same bit for bit.
+ *   "xstage"      x windows in LDS: 1 (default) each row block loads the
+ *                 x window its columns span into LDS once and reads x from
+ *                 there, in the forward product (FP32, one right-hand side)
+ *                 and the fused residual (FP32, FP64), for shares whose
+ *                 blocks span at most half as many columns as entries
+ *                 (bands; probed once per plan and colind, with one wait on
+ *                 the stream); 0 never; 2 always (a window wider than the
+ *                 LDS buffer reads the rest from memory). Results are the
+ *                 same bit for bit.
  *   "gemm_tile"   drx_gemm workgroup tile: 1 128 x 128; 2 256 x 128; 3 and
  *                 4 the same with stages twice as deep in k; 5 256 x 256;
  *                 6 and 7 128 x 128 and 256 x 128 loading stages straight

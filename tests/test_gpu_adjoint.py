@@ -135,6 +135,33 @@ def test_fused_residual_matches_separate(orc, dtype):
     assert np.abs(dv.cpu().numpy() - ref).max() <= (1e-15 if dtype == torch.float64 else 1e-6) * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("kind", ["band", "wide_band", "random"])
+def test_fused_residual_x_staging_bitexact(orc, dtype, kind):
+    """xstage = 2 (each block's x window staged in LDS; windows wider than
+    the staging buffer read the rest from memory) and the default give the
+    same bits as per-entry gathers (xstage = 0)."""
+    if kind == "band":
+        A = drx.gen.band(60000, 16, 16, vdtype=dtype, seed=31)
+    elif kind == "wide_band":
+        A = drx.gen.band(60000, 400, 400, vdtype=dtype, seed=32)
+    else:
+        A = drx.gen.zipf_csr(40000, 50000, s=1.5, lmax=300, seed=33, vdtype=dtype)
+    x = drx.gen.uniform(A.shape[1], dtype, -1, 1, seed=34)
+    t = drx.gen.uniform(A.shape[0], dtype, -1, 1, seed=35)
+    outs = {}
+    for xs in (0, 2, 1):
+        with drx.tuned(xstage=xs):
+            outs[xs] = A.fused_residual(x, t)
+    torch.cuda.synchronize()
+    for xs in (2, 1):
+        for a, b in zip(outs[0], outs[xs]):
+            assert torch.equal(a, b), (kind, xs)
+    rp, ci, v = A.rowptr.cpu().numpy(), A.colind.cpu().numpy(), A.vals.cpu().numpy()
+    ystar, absy = orc.spmv(rp, ci, v, x.cpu().numpy())
+    assert componentwise_err(outs[2][0].cpu().numpy(), ystar, absy) < (1e-12 if dtype == torch.float64 else 1e-5)
+
+
 def test_autograd_finite_differences_fp64():
     """Banded 4096-row instance (band +-16, 33 diagonals), loss
     0.5 ||A x - t||^2: the autograd gradients (drx adjoint + dvals kernels)
