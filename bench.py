@@ -448,6 +448,9 @@ def main():
     ap.add_argument("--forms", default="all", help="all | none | comma list")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--xstage", type=int, default=None,
+                    help="x windows in LDS (the library's xstage knob: 0 never, 1 default, "
+                         "2 always); for profiling the forms with and without")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 2 GiB device copy 5 times (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -467,6 +470,8 @@ def main():
               % args.gpus, file=sys.stderr)
         sys.exit(2)
     torch.cuda.set_device(0)
+    if args.xstage is not None:
+        drx.tune_set("xstage", args.xstage)
     A, x, y, B, wall_ms, ev_ms, layout, gather = headline(args.steps, args.warmup)
     value = B / (wall_ms * 1e-3) / 1e9
     achieved = B / (ev_ms * 1e-3) / 1e9
