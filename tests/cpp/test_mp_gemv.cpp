@@ -132,7 +132,7 @@ TEST(SparseMatrix, GemvEqBenchInputExact) {
   dr::destroy_csr_matrix_view(csr);
 }
 
-// int32 indices (the BASELINE shapes use int32) and empty rows, mixed signs.
+// int32 indices (as the benchmark workloads use) and empty rows, mixed signs.
 TEST(SparseMatrix, Int32IndicesEmptyRows) {
   const int m = 500, n = 300;
   std::vector<std::pair<std::pair<int, int>, double>> cells;
