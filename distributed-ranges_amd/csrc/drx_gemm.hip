@@ -248,7 +248,7 @@ __global__ void __launch_bounds__(64 * WM * WN)
 // products, so that those registers are never live beside the operands.
 __device__ __forceinline__ int dl_fa(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
 // One 16-byte global -> LDS copy per lane into the wave's 1 KB block at lds.
-__device__ __forceinline__ void lds_dma16(const float *src, char *lds) {
+__device__ __forceinline__ void lds_dma16(const void *src, char *lds) {
   __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)lds, 16, 0, 0);
 }
 __device__ __forceinline__ int dl_hb(int k) { return 4 * ((k >> 2) & 3); }
@@ -416,6 +416,186 @@ __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_
   }
 }
 
+// FP64 direct-to-LDS variant, BK = 16, BN = 128: A stage rows of 128 bytes
+// (8 slots; a wave's 1 KB block holds 8 rows), B stage rows of 1 KB (one per
+// block). Slot c of A row r (r mod 16 within its MFMA tile) sits at
+// c ^ fa64(r); slot c of B row k at c ^ hb64(k). The 16-byte A reads (two
+// k per lane) and the 8-byte B reads stay free of bank conflicts for 8 or
+// 16 lanes per 128 or 256 bytes.
+__device__ __forceinline__ int dl64_fa(int r) { return (r ^ (r >> 3)) & 7; }
+__device__ __forceinline__ int dl64_hb(int k) { return 8 * ((k >> 1) & 1); }
+
+template <int BM, int WM, int WN>
+__global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2)))
+    k_gemm_dl64(int64_t m, int64_t n, int64_t K, double alpha, const double *__restrict__ A,
+                int64_t lda, const double *__restrict__ B, int64_t ldb, double beta,
+                double *__restrict__ C, int64_t ldc, int64_t tiles_m, int64_t tiles_n,
+                int64_t per_xcd, int vec_ok) {
+  typedef double f64x2 __attribute__((ext_vector_type(2)));
+  constexpr int BN = 128, BK = 16, NW = WM * WN;
+  constexpr int A_BYTES = BM * BK * 8, B_BYTES = BK * BN * 8, STAGE = A_BYTES + B_BYTES;
+  constexpr int A_INSTR = A_BYTES / 1024 / NW, B_INSTR = B_BYTES / 1024 / NW;
+  constexpr int TM = BM / WM, TN = BN / WN, FM = TM / 16, FN = TN / 16;
+  static_assert(A_INSTR * NW * 1024 == A_BYTES && B_INSTR * NW * 1024 == B_BYTES,
+                "stages must split into whole 1 KB wave blocks");
+  static_assert(TM % 16 == 0 && TN % 16 == 0, "wave tile must be whole MFMA tiles");
+  __shared__ __attribute__((aligned(1024))) char smem[2 * STAGE];
+
+  const int64_t logical = map_block(blockIdx.x, per_xcd);
+  if (logical >= tiles_m * tiles_n) return;  // whole workgroup leaves together
+  const int64_t per_group = int64_t(kGroupM) * tiles_n;
+  const int64_t first = logical / per_group * kGroupM;
+  const int64_t gsz = tiles_m - first < kGroupM ? tiles_m - first : kGroupM;
+  const int64_t in_g = logical % per_group;
+  const int64_t m0 = (first + in_g % gsz) * BM;
+  const int64_t n0 = (in_g / gsz) * BN;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int g = lane >> 4, r = lane & 15;
+  const int wm = wave / WN, wn = wave % WN;
+  const bool n_inner = vec_ok && n0 + BN <= n;
+
+  // A block p * NW + wave: rows block*8 + lane/8, slot (lane%8) ^ fa64(row);
+  // B block p * NW + wave: row = block, slot lane ^ hb64(row).
+  auto a_row = [&](int p) { return (p * NW + wave) * 8 + (lane >> 3); };
+  auto a_col = [&](int p) { return 2 * ((lane & 7) ^ dl64_fa(a_row(p) & 15)); };
+  auto b_row = [&](int p) { return p * NW + wave; };
+  auto b_col = [&](int p) { return 2 * (lane ^ dl64_hb(b_row(p))); };
+  const double *a_src[A_INSTR];
+  const double *b_src[B_INSTR];
+#pragma unroll
+  for (int p = 0; p < A_INSTR; ++p) {
+    const int64_t gi = m0 + a_row(p) < m ? m0 + a_row(p) : m - 1;
+    a_src[p] = A + gi * lda + a_col(p);
+  }
+#pragma unroll
+  for (int p = 0; p < B_INSTR; ++p) b_src[p] = B + b_row(p) * ldb + n0 + b_col(p);
+
+  auto issue_direct = [&](int64_t k0, char *st) {
+#pragma unroll
+    for (int p = 0; p < A_INSTR; ++p) lds_dma16(a_src[p] + k0, st + (p * NW + wave) * 1024);
+#pragma unroll
+    for (int p = 0; p < B_INSTR; ++p)
+      lds_dma16(b_src[p] + k0 * ldb, st + A_BYTES + (p * NW + wave) * 1024);
+  };
+  auto load_store_regs = [&](int64_t k0, char *st) {
+#pragma unroll
+    for (int p = 0; p < A_INSTR; ++p) {
+      const int64_t gi = m0 + a_row(p), gk = k0 + a_col(p);
+      f64x2 v;
+      if (vec_ok && gi < m && gk + 2 <= K) {
+        v = *reinterpret_cast<const f64x2 *>(A + gi * lda + gk);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 2; ++e) v[e] = (gi < m && gk + e < K) ? A[gi * lda + gk + e] : 0.0;
+      }
+      *reinterpret_cast<f64x2 *>(st + (p * NW + wave) * 1024 + lane * 16) = v;
+    }
+#pragma unroll
+    for (int p = 0; p < B_INSTR; ++p) {
+      const int64_t gk = k0 + b_row(p), gj = n0 + b_col(p);
+      f64x2 v;
+      if (vec_ok && gk < K && gj + 2 <= n) {
+        v = *reinterpret_cast<const f64x2 *>(B + gk * ldb + gj);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 2; ++e) v[e] = (gk < K && gj + e < n) ? B[gk * ldb + gj + e] : 0.0;
+      }
+      *reinterpret_cast<f64x2 *>(st + A_BYTES + (p * NW + wave) * 1024 + lane * 16) = v;
+    }
+  };
+
+  // Operand offsets (bytes within a stage). A row block i of the wave at
+  // a_off[s] + 2048 i for sub-step s (slot 4 s + g, i.e. bit 2 flipped for
+  // s = 1); B row 8 s + 2 g + kk of column tile j at b_off[j] + 1024 (8 s + kk)
+  // (hb64 is the same for all of lane group g's rows).
+  const int a_base = (wm * TM + r) * 128;
+  const int a_off0 = a_base + ((g ^ dl64_fa(r)) * 16);
+  const int a_off1 = a_base + (((g ^ dl64_fa(r)) ^ 4) * 16);
+  int b_off[FN];
+#pragma unroll
+  for (int j = 0; j < FN; ++j) {
+    const int col = wn * TN + j * 16 + r;
+    b_off[j] = A_BYTES + 2 * g * 1024 + (((col >> 1) ^ dl64_hb(2 * g)) * 16) + (col & 1) * 8;
+  }
+
+  f64x4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = f64x4{};
+
+  const int64_t nk = (K + BK - 1) / BK;
+  if (nk > 0) {
+    if (n_inner && BK <= K)
+      issue_direct(0, smem);
+    else
+      load_store_regs(0, smem);
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    const char *cur = smem + (kt & 1) * STAGE;
+    char *nxt = smem + ((kt + 1) & 1) * STAGE;
+    const bool more = kt + 1 < nk;
+    const bool direct = more && n_inner && (kt + 2) * BK <= K;
+    if (direct) issue_direct((kt + 1) * BK, nxt);
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2) {
+      f64x2 a[FM];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+        a[i] = *reinterpret_cast<const f64x2 *>(cur + (s2 ? a_off1 : a_off0) + i * 2048);
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk) {
+        double b[FN];
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          b[j] = *reinterpret_cast<const double *>(cur + b_off[j] + (8 * s2 + kk) * 1024);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<double>::op(a[i][kk], b[j], acc[i][j]);
+      }
+    }
+    if (more && !direct) load_store_regs((kt + 1) * BK, nxt);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t gi = m0 + wm * TM + i * 16 + Mfma<double>::row(g, v);
+      if (gi >= m) continue;
+      double *crow = C + gi * ldc;
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int64_t gj = n0 + wn * TN + j * 16 + r;
+        if (gj >= n) continue;
+        const double prod = alpha * acc[i][j][v];
+        crow[gj] = beta == 0.0 ? prod : prod + beta * crow[gj];
+      }
+    }
+  }
+}
+
+inline void launch_gemm_dl64(int64_t m, int64_t n, int64_t k, double alpha, const double *a,
+                             int64_t lda, const double *b, int64_t ldb, double beta, double *c,
+                             int64_t ldc, hipStream_t s) {
+  const int64_t tiles_m = (m + 127) / 128, tiles_n = (n + 127) / 128;
+  const int64_t per_xcd = (tiles_m * tiles_n + kXcds - 1) / kXcds;
+  DRX_REQUIRE(per_xcd * kXcds <= int64_t(0xffffffff), "product too large for one launch");
+  const bool aligned = reinterpret_cast<uintptr_t>(a) % 16 == 0 &&
+                       reinterpret_cast<uintptr_t>(b) % 16 == 0 && lda % 2 == 0 && ldb % 2 == 0;
+  hipLaunchKernelGGL((k_gemm_dl64<128, 2, 2>), dim3(static_cast<unsigned>(per_xcd * kXcds)),
+                     dim3(256), 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tiles_m,
+                     tiles_n, per_xcd, aligned ? 1 : 0);
+  DRX_HIP(hipGetLastError());
+}
+
 template <int BM, int WM, int WN>
 void launch_gemm_dl(int64_t m, int64_t n, int64_t k, float alpha, const float *a, int64_t lda,
                     const float *b, int64_t ldb, float beta, float *c, int64_t ldc,
@@ -453,13 +633,14 @@ void launch_gemm(int64_t m, int64_t n, int64_t k, T alpha, const T *a, int64_t l
 //   2: 256 x 128, eight waves of 64 x 64, same depth
 //   3: 128 x 128, twice the depth          4: 256 x 128, twice the depth
 //   5: 256 x 256, eight waves of 64 x 128 (FP32 only; FP64 takes 4)
-//   6, 7: 128 x 128 and 256 x 128 loading stages straight into LDS (FP32
-//      only; FP64 takes 4)
-// 0 picks by type and size (measured on 4096^3 .. 16384^3): FP64 takes 3;
-// FP32 takes 6 once C holds 4096 tiles of 128 x 128 (about 8192^2), 5 while
-// it holds at least 256 of 256 x 256 (about 4096^2), else 1.
+//   6, 7: 128 x 128 and 256 x 128 loading stages straight into LDS (FP32;
+//      FP64 has the 128 x 128 one, BK 16, and takes 4 for 7)
+// 0 picks by type and size (measured on 4096^3 .. 16384^3): FP64 takes 6
+// (its direct-to-LDS kernel: 64-68 TFLOP/s against 57-65 for 3); FP32 takes
+// 6 once C holds 4096 tiles of 128 x 128 (about 8192^2), 5 while it holds at
+// least 256 of 256 x 256 (about 4096^2), else 1.
 inline int default_gemm_tile(bool fp64, int64_t m, int64_t n) {
-  if (fp64) return 3;
+  if (fp64) return 6;
   if (((m + 127) / 128) * ((n + 127) / 128) >= 4096) return 6;
   return ((m + 255) / 256) * ((n + 255) / 256) >= 256 ? 5 : 1;
 }
@@ -481,9 +662,11 @@ void run_gemm(int64_t m, int64_t n, int64_t k, T alpha, const T *a, int64_t lda,
           launch_gemm_dl<128, 2, 2>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s);
         else
           launch_gemm_dl<256, 4, 2>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s);
-        break;
+      } else if (t == 6) {
+        launch_gemm_dl64(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s);
+      } else {
+        launch_gemm<T, 256, 128, 2 * BK, 4, 2>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s);
       }
-      launch_gemm<T, 256, 128, 2 * BK, 4, 2>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s);
       break;
     case 5:
       if constexpr (sizeof(T) == 4) {

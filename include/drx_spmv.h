@@ -102,13 +102,13 @@ const char *drx_last_error(void);
  *                 LDS buffer reads the rest from memory). Results are the
  *                 same bit for bit.
  *   "gemm_tile"   drx_gemm workgroup tile: 1 128 x 128; 2 256 x 128; 3 and
- *                 4 the same with stages twice as deep in k; 5 256 x 256;
- *                 6 and 7 128 x 128 and 256 x 128 loading stages straight
- *                 into LDS (5..7 are FP32 kernels; FP64 takes 4 for them);
- *                 0 (default) by type and size: FP64 3; FP32 6 once C
- *                 holds 4096 tiles of 128 x 128 (about 8192^2), 5 while it
- *                 holds at least 256 tiles of 256 x 256 (about 4096^2),
- *                 else 1. */
+ *                 4 the same with stages twice as deep in k; 5 256 x 256
+ *                 (FP32; FP64 takes 4); 6 128 x 128 loading stages straight
+ *                 into LDS (FP32 and FP64); 7 256 x 128 the same way (FP32;
+ *                 FP64 takes 4); 0 (default) by type and size: FP64 6; FP32
+ *                 6 once C holds 4096 tiles of 128 x 128 (about 8192^2), 5
+ *                 while it holds at least 256 tiles of 256 x 256 (about
+ *                 4096^2), else 1. */
 int drx_tune_set(const char *key, int64_t value);
 int drx_tune_get(const char *key, int64_t *value);
 
