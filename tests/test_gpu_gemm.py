@@ -143,9 +143,10 @@ def test_gemm_reference_example_inputs(dtype):
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_gemm_large_against_fp64_torch(dtype):
-    """4096^3 (FP32 on 256 x 256 tiles, FP64 on deep 128 x 128 ones: the
-    defaults there) against torch's float64 product of the same inputs,
-    within the componentwise bound (|A||B| from float64 too)."""
+    """4096^3 (FP32 on 256 x 256 tiles, FP64 on 128 x 128 tiles loaded
+    straight into LDS: the defaults there) against torch's float64 product
+    of the same inputs, within the componentwise bound (|A||B| from float64
+    too)."""
     n = 4096
     g = torch.Generator(device=DEV).manual_seed(1)
     a = torch.randn(n, n, device=DEV, dtype=dtype, generator=g)
