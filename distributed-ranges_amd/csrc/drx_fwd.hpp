@@ -404,9 +404,10 @@ template <int WG, int CAP, int MODE, typename TA, typename TI> constexpr int fwd
 template <int WG, int CAP, int MODE, bool BUFX, typename TV, typename TX,
           typename TY, typename TA, typename TI, int VW = 1, bool XS = false>
 __global__ __launch_bounds__(WG)
-// (16-bit values take one wave fewer: at the LDS-bound count their pair
-// unpacking spilled registers; 3-D Laplacian BF16 390 -> 371 us.)
-__attribute__((amdgpu_waves_per_eu(fwd_waves<WG, CAP, MODE, TA, TI>() - (sizeof(TV) == 2 ? 1 : 0), 8)))
+// (16-bit values in plain mode take one wave fewer: at the LDS-bound count
+// their pair unpacking spilled registers; 3-D Laplacian BF16 390 -> 371 us.
+// The panel kernel's LDS already holds it at 7 waves, where it does not.)
+__attribute__((amdgpu_waves_per_eu(fwd_waves<WG, CAP, MODE, TA, TI>() - (sizeof(TV) == 2 && MODE == kPlain ? 1 : 0), 8)))
 void k_spmv_fwd(FwdArgs a) {
   __shared__ FwdSmem<WG, CAP, MODE == kRowMap, TA, TI> sm;
   for (int64_t wi = blockIdx.x; wi < a.per_xcd * kXcds; wi += a.grid) {
