@@ -406,7 +406,7 @@ static void replan(Batch &b, int64_t block_nnz, hipStream_t s) {
     a.perm = it.perm;
     a.x = it.x;
     a.ldx = it.ldx;
-    a.xbytes = static_cast<uint32_t>(std::max<int64_t>(p->ncols, 1) * xs);
+    a.xbytes = p->nnz > 0 ? static_cast<uint32_t>(std::max<int64_t>(p->ncols, 1) * xs) : 0;
     a.y = it.y;
     a.ldy = it.ldy;
     a.k = b.k;

@@ -63,8 +63,8 @@ __device__ __forceinline__ void fill_rowid(const DvalsArgs &a, const int *rp_s,
 // registers, each entry's row is looked up in LDS (rowid), dy of the
 // block's rows is staged in LDS, and dvals is written with coalesced stores.
 // A last row longer than the window is done by the whole workgroup.
-// (Staging x windows in LDS, which speeds the forward and fused kernels over
-// bands, measured slower here: band +-16 FP32 540 vs 431 us.)
+// (Staging x windows in LDS measured slower here: band +-16 FP32 540 vs
+// 431 us.)
 template <int WG, int CAP, typename TX, typename TD, typename TA, typename TI>
 __global__ __launch_bounds__(WG) void k_dvals(DvalsArgs a) {
   constexpr int T = CAP / WG;
@@ -121,6 +121,9 @@ __global__ __launch_bounds__(WG) void k_dvals(DvalsArgs a) {
         for (int i = tid; i < nr; i += WG) dy_s[i] = to_acc<TA>(dyj[i]);
       }
       __syncthreads();
+      // (Issuing all x reads before the products, as the forward and fused
+      // kernels do, needs 2T more registers here: one wave fewer, band
+      // +-16 FP32 430 -> 461 us.)
 #pragma unroll
       for (int u = 0; u < T; u++) {
         const int idx = tid + u * WG;
@@ -222,19 +225,23 @@ __global__ __launch_bounds__(kWG) void k_fused_residual(FusedArgs a) {
     const int64_t last = r1 - 1;
     const int64_t rs_end = b.es == e1 ? r1 : last;
     const int n_s = static_cast<int>(b.es - b.e0);
-    if constexpr (XS) {
-      uint32_t c[T];
-      TA w[T];
+    // The window is streamed with range-checked buffer loads (slots past
+    // n_s read 0), and every lane issues all of its x reads before it stores
+    // a product: per-entry predicates made each wave wait for one load at a
+    // time. Slots past n_s read x[0] and store products nothing reads.
+    uint32_t c[T];
+    TA w[T], xv[T];
+    {
+      const rsrc_t rv = make_rsrc(vals + b.e0, n_s * sizeof(TV));
+      const rsrc_t rc = make_rsrc(colind + b.e0, n_s * sizeof(TI));
 #pragma unroll
       for (int u = 0; u < T; u++) {
-        const int idx = tid + u * kWG;
-        c[u] = 0;
-        w[u] = 0;
-        if (idx < n_s) {
-          w[u] = to_acc<TA>(ld_stream(vals + b.e0 + idx));
-          c[u] = col32(ld_stream(colind + b.e0 + idx));
-        }
+        const uint32_t idx = tid + u * kWG;
+        w[u] = to_acc<TA>(buf_load<TV, kNT>(rv, idx * sizeof(TV)));
+        c[u] = col32(buf_load<TI, kNT>(rc, idx * sizeof(TI)));
       }
+    }
+    if constexpr (XS) {
       uint32_t lo, hi, xn = 0;
       block_col_range<kWG, T>(c, n_s, xmm, lo, hi);
       if (lo <= hi) {
@@ -242,27 +249,30 @@ __global__ __launch_bounds__(kWG) void k_fused_residual(FusedArgs a) {
         for (uint32_t i = tid; i < xn; i += kWG) xs[i] = x[lo + i];
       }
       __syncthreads();
+      bool far = false;
 #pragma unroll
       for (int u = 0; u < T; u++) {
-        const int idx = tid + u * kWG;
-        if (idx < n_s) {
-          const uint32_t off = c[u] - lo;
-          const TA xv = to_acc<TA>(off < xn ? xs[off] : x[c[u]]);
-          xg[idx] = xv;
-          prod[idx] = w[u] * xv;
-        }
+        const uint32_t off = c[u] - lo;
+        xv[u] = to_acc<TA>(xs[off < xn ? off : 0]);
+        far |= off >= xn && tid + u * kWG < n_s;
       }
+      if (far) {
+#pragma unroll
+        for (int u = 0; u < T; u++)
+          if (c[u] - lo >= xn && tid + u * kWG < n_s) xv[u] = to_acc<TA>(x[c[u]]);
+      }
+    } else if (n_s > 0) {
+#pragma unroll
+      for (int u = 0; u < T; u++) xv[u] = to_acc<TA>(x[c[u]]);
     } else {
 #pragma unroll
-      for (int u = 0; u < T; u++) {
-        const int idx = tid + u * kWG;
-        if (idx < n_s) {
-          const TA w = to_acc<TA>(ld_stream(vals + b.e0 + idx));
-          const TA xv = to_acc<TA>(x[static_cast<int64_t>(ld_stream(colind + b.e0 + idx))]);
-          xg[idx] = xv;
-          prod[idx] = w * xv;
-        }
-      }
+      for (int u = 0; u < T; u++) xv[u] = 0;
+    }
+#pragma unroll
+    for (int u = 0; u < T; u++) {
+      const int idx = tid + u * kWG;
+      xg[idx] = xv[u];
+      prod[idx] = w[u] * xv[u];
     }
     __syncthreads();
     const int64_t nr = rs_end - b.r0;
@@ -409,12 +419,11 @@ void run_fused(const Plan &p, int vtype, const void *colind, const void *vals,
   a.y = y;
   a.r = r;
   a.dvals = dvals;
-  // Bands: stage each block's x window in LDS (FP32 and FP64; band +-16:
-  // FP32 718 -> 512 us, FP64 1247 -> 877 us).
-  const int xsk = tuned_xstage();
-  const bool fx = (vtype == DRX_F32 && xtype == DRX_F32) || (vtype == DRX_F64 && xtype == DRX_F64);
-  a.xstage = p.nnz > 0 && xsk != 0 && static_cast<uint64_t>(p.ncols) < (uint64_t(1) << 32) &&
-             (xsk == 2 || (fx && !p.owns_async && (column_locality(p, colind, s) & 4) != 0));
+  // x windows staged in LDS only when the knob forces it: with all of a
+  // lane's x reads in flight, per-entry reads win on bands (+-16 FP32:
+  // 498 vs 561 us per call).
+  a.xstage = p.nnz > 0 && tuned_xstage() == 2 &&
+             static_cast<uint64_t>(p.ncols) < (uint64_t(1) << 32);
   const dim3 grid(static_cast<unsigned>(a.grid)), block(kWG);
   const bool i32 = p.itype == DRX_I32;
   const bool small = p.cap == 1024;

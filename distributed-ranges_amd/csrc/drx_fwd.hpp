@@ -248,13 +248,26 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
     } else {
       const rsrc_t rc = make_rsrc(colind + b.e0, n_s * sizeof(TI));
       if constexpr (MODE == kPerm) {
+        // Slots past the window read perm = 0, so every lane loads vals[0]
+        // or its own entry unconditionally (a predicated load made each
+        // wave wait for one value at a time); n_s > 0 guarantees vals[0].
         const rsrc_t rp = make_rsrc(perm + b.e0, n_s * sizeof(TI));
+        int64_t pe[T];
 #pragma unroll
         for (int u = 0; u < T; u++) {
           const uint32_t idx = tid + u * WG;
           c[u] = col32(buf_load<TI, kNT>(rc, idx * sizeof(TI)));
-          const int64_t pe = static_cast<int64_t>(buf_load<TI, kNT>(rp, idx * sizeof(TI)));
-          v[u] = idx < static_cast<uint32_t>(n_s) ? to_acc<TA>(vals[pe]) : TA(0);
+          pe[u] = static_cast<int64_t>(buf_load<TI, kNT>(rp, idx * sizeof(TI)));
+        }
+        if (n_s > 0) {
+#pragma unroll
+          for (int u = 0; u < T; u++) v[u] = to_acc<TA>(vals[pe[u]]);
+#pragma unroll
+          for (int u = 0; u < T; u++)
+            v[u] = tid + u * WG < static_cast<uint32_t>(n_s) ? v[u] : TA(0);
+        } else {
+#pragma unroll
+          for (int u = 0; u < T; u++) v[u] = 0;
         }
       } else {
         const rsrc_t rv = make_rsrc(vals + b.e0, n_s * sizeof(TV));
@@ -297,22 +310,25 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
           const rsrc_t rx = make_rsrc(x, a.xbytes);
           for (uint32_t i = tid; i < xn; i += WG) xs[i] = buf_load<TX>(rx, (lo + i) * sizeof(TX));
           __syncthreads();
+          // All LDS reads first, then the rare columns past the window from
+          // memory, then the products (slots past n_s hold 0 * x[lo]).
+          TA xv[T];
+          bool far = false;
 #pragma unroll
           for (int u = 0; u < T; u++) {
-            if (tid + u * WG < n_s) {
-              const uint32_t off = c[u] - lo;
-              TA xv;
-              if (off < xn)
-                xv = to_acc<TA>(xs[off]);
-              else
-                xv = to_acc<TA>(buf_load<TX>(rx, c[u] * sizeof(TX)));
-              v[u] = v[u] * xv;
-            }
+            const uint32_t off = c[u] - lo;
+            xv[u] = to_acc<TA>(xs[off < xn ? off : 0]);
+            far |= off >= xn && tid + u * WG < n_s;
+          }
+          if (far) {
+#pragma unroll
+            for (int u = 0; u < T; u++)
+              if (c[u] - lo >= xn && tid + u * WG < n_s)
+                xv[u] = to_acc<TA>(buf_load<TX>(rx, c[u] * sizeof(TX)));
           }
           __syncthreads();
 #pragma unroll
-          for (int u = 0; u < T; u++)
-            if (tid + u * WG < n_s) prod[tid + u * WG] = v[u];
+          for (int u = 0; u < T; u++) prod[tid + u * WG] = v[u] * xv[u];
           staged = true;
         }
       }
@@ -323,11 +339,30 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
       if (staged) {
       } else if constexpr (BUFX) {
         const rsrc_t rx = make_rsrc(xj, a.xbytes);
+        if constexpr (sizeof(TA) == 8) {
+          // 8-byte sums: one gather in flight per entry slot. Holding all T
+          // gathered values costs T more register pairs, which spill at 7
+          // waves or cost a wave (3-D Laplacian 256^3 FP64: 478 us this way,
+          // 492-652 us with all gathers in flight).
 #pragma unroll
-        for (int u = 0; u < T; u++) {
-          const int idx = eidx(u);
-          if (idx >= 0 && idx < n_s)
-            prod[idx] = v[u] * to_acc<TA>(buf_load<TX>(rx, c[u] * sizeof(TX)));
+          for (int u = 0; u < T; u++) {
+            const int idx = eidx(u);
+            if (idx >= 0 && idx < n_s)
+              prod[idx] = v[u] * to_acc<TA>(buf_load<TX>(rx, c[u] * sizeof(TX)));
+          }
+        } else {
+          // Every lane issues all T gathers before storing a product: a
+          // per-entry predicate around load and store made the compiler
+          // wait for each gather in turn (band +-16 FP32: 338 -> 273 us).
+          // Slots past the window gather x[0] (their column reads 0) into
+          // product slots that the reduction never reads; the pair slot
+          // before a window (idx = -1, lane 0) goes to CAP - 1, which only
+          // the tail entry -- stored after it by the same lane -- uses.
+          TA xg[T];
+#pragma unroll
+          for (int u = 0; u < T; u++) xg[u] = to_acc<TA>(buf_load<TX>(rx, c[u] * sizeof(TX)));
+#pragma unroll
+          for (int u = 0; u < T; u++) prod[(eidx(u) + CAP) & (CAP - 1)] = v[u] * xg[u];
         }
         if (tail && tid == 0) prod[CAP - 1] = v_tail * to_acc<TA>(buf_load<TX>(rx, c_tail * sizeof(TX)));
       } else {

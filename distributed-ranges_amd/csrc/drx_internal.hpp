@@ -99,8 +99,7 @@ void scratch_free(void *p, hipStream_t s);
 int mrhs_kp(int64_t k, int xtype, bool loose_local = false);
 // Column locality of the share: bit 0 when most row blocks' windows span few
 // more columns than entries (banded, 2-D stencils), bit 1 when they span at
-// most ncols / 16 (3-D stencils too), bit 2 when they span at most half as
-// many columns as entries (bands: one staged x window serves many entries). Measured once per colind pointer (one
+// most ncols / 16 (3-D stencils too). Measured once per colind pointer (one
 // small kernel and a wait), cached in the plan; 0 while the stream is being
 // captured.
 int column_locality(const Plan &p, const void *colind, hipStream_t s);

@@ -64,16 +64,15 @@ void scratch_free(void *p, hipStream_t s) {
 
 // Samples up to 4096 row blocks: the span between the first and the last
 // column index of each window, against its entry count (tight: at most
-// 4 n + 64; narrow: at most n / 2) and against ncols / 16 (loose).
-// out = {tight, loose, sampled, narrow}.
+// 4 n + 64) and against ncols / 16 (loose). out = {tight, loose, sampled}.
 template <typename TI>
 __global__ void k_col_local(const BlockRec *rec, int64_t nblocks, int64_t ncols,
                             const TI *colind, int *out) {
-  __shared__ int cnt[4];
-  if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
+  __shared__ int cnt[3];
+  if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
   __syncthreads();
   const int64_t S = nblocks < 4096 ? nblocks : 4096;
-  int tight = 0, loose = 0, seen = 0, narrow = 0;
+  int tight = 0, loose = 0, seen = 0;
   for (int64_t i = threadIdx.x; i < S; i += blockDim.x) {
     const BlockRec b = rec[i * nblocks / S];
     const int64_t n = b.es - b.e0;
@@ -84,19 +83,16 @@ __global__ void k_col_local(const BlockRec *rec, int64_t nblocks, int64_t ncols,
       seen++;
       if (span <= 4 * n + 64) tight++;
       if (span <= 4 * n + 64 || span <= ncols / 16) loose++;
-      if (span <= n / 2) narrow++;
     }
   }
   atomicAdd(&cnt[0], tight);
   atomicAdd(&cnt[1], loose);
   atomicAdd(&cnt[2], seen);
-  atomicAdd(&cnt[3], narrow);
   __syncthreads();
   if (threadIdx.x == 0) {
     out[0] = cnt[0];
     out[1] = cnt[1];
     out[2] = cnt[2];
-    out[3] = cnt[3];
   }
 }
 
@@ -109,8 +105,8 @@ int column_locality(const Plan &p, const void *colind, hipStream_t s) {
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
   DRX_HIP(hipStreamIsCapturing(s, &st));
   if (st != hipStreamCaptureStatusNone) return 0;
-  int *d = static_cast<int *>(scratch_alloc(4 * sizeof(int), s));
-  int h[4] = {0, 0, 0, 0};
+  int *d = static_cast<int *>(scratch_alloc(3 * sizeof(int), s));
+  int h[3] = {0, 0, 0};
   try {
     if (p.itype == DRX_I32)
       hipLaunchKernelGGL(k_col_local<int32_t>, dim3(1), dim3(256), 0, s, p.rec, p.nblocks,
@@ -129,7 +125,6 @@ int column_locality(const Plan &p, const void *colind, hipStream_t s) {
   int loc = 0;
   if (h[2] > 0 && 10 * h[0] >= 9 * h[2]) loc |= 1;
   if (h[2] > 0 && 10 * h[1] >= 9 * h[2]) loc |= 2;
-  if (h[2] > 0 && 10 * h[3] >= 9 * h[2]) loc |= 4;
   // Invalidate, then publish pointer and value.
   p.loc_local.store(-1, std::memory_order_release);
   p.loc_colind.store(colind, std::memory_order_release);

@@ -172,7 +172,10 @@ static void run_spmv_impl(const Plan &p, int vtype, const void *colind,
   const uint64_t xb =
       static_cast<uint64_t>(std::max<int64_t>(p.ncols, 1)) * dtype_size(xtype);
   const bool bufx = xb < (uint64_t(1) << 32);
-  a.xbytes = bufx ? static_cast<uint32_t>(xb) : 0;
+  // Gathers are range-checked against xbytes, and lanes past a window
+  // gather column 0: a share without entries (x may be null or empty) gets
+  // an empty range.
+  a.xbytes = bufx && p.nnz > 0 ? static_cast<uint32_t>(xb) : 0;
   a.ncols = p.ncols;
   if constexpr (MODE != kRowMap) {
     // Row-major chunks of x for k >= 2, except up to 4 right-hand sides over
@@ -189,18 +192,11 @@ static void run_spmv_impl(const Plan &p, int vtype, const void *colind,
       return;
     }
   }
-  // One right-hand side over row blocks whose columns span at most half as
-  // many columns as entries (bands): stage each block's x window in LDS
-  // instead of gathering per entry. By default FP32 only: FP32 bands gain
-  // 12-20 % (+-16: 307 -> 271 us), FP64 ones do not (467 vs 470 us), and
-  // wider windows (2-D Laplacian, 2.4 columns per entry) lose. Decided once
-  // per plan and colind (the locality probe waits on the stream once); not
-  // for the single-call plans of drx_spmv_csr.
+  // Staging each block's x window in LDS (one right-hand side) only when
+  // the knob forces it: with all of a lane's gathers in flight, per-entry
+  // gathers beat it on bands (+-16 FP32: 241 vs 286 us per call).
   if constexpr (MODE == kPlain) {
-    const int xsk = tuned_xstage();
-    if (k == 1 && bufx && xsk != 0 && p.nnz > 0)
-      a.xstage = xsk == 2 || (xtype == DRX_F32 && vtype == DRX_F32 && !p.owns_async &&
-                              (column_locality(p, colind, s) & 4) != 0);
+    if (k == 1 && bufx && p.nnz > 0) a.xstage = tuned_xstage() == 2;
   }
   if (vtype == DRX_F64 && xtype == DRX_F64 && ytype == DRX_F64)
     launch_fwd_t<MODE, double, double, double, double>(p, a, bufx, s);

@@ -92,15 +92,13 @@ const char *drx_last_error(void);
  *                 is the default 1024; 0: the plan's own blocks.
  *   "grid_cap"    most workgroups of a per-share launch (0 = one per row
  *                 block); capped launches loop over the blocks.
- *   "xstage"      x windows in LDS: 1 (default) each row block loads the
- *                 x window its columns span into LDS once and reads x from
- *                 there, in the forward product (FP32, one right-hand side)
- *                 and the fused residual (FP32, FP64), for shares whose
- *                 blocks span at most half as many columns as entries
- *                 (bands; probed once per plan and colind, with one wait on
- *                 the stream); 0 never; 2 always (a window wider than the
- *                 LDS buffer reads the rest from memory). Results are the
- *                 same bit for bit.
+ *   "xstage"      x windows in LDS: 2 makes each row block of the forward
+ *                 product (one right-hand side) and of the fused residual
+ *                 load the x window its columns span into LDS once and read
+ *                 x from there (a window wider than the LDS buffer reads the
+ *                 rest from memory); 0 and 1 (default) gather x per entry,
+ *                 which measured faster on bands. Results are the same bit
+ *                 for bit.
  *   "gemm_tile"   drx_gemm workgroup tile: 1 128 x 128; 2 256 x 128; 3 and
  *                 4 the same with stages twice as deep in k; 5 256 x 256
  *                 (FP32; FP64 takes 4); 6 128 x 128 loading stages straight

@@ -633,9 +633,8 @@ def _staging_cases(rng):
 @pytest.mark.parametrize("itype", [torch.int32, torch.int64])
 def test_x_staging_bitexact(orc, vdt, itype):
     """xstage = 2 (every block stages its x window in LDS) gives the same
-    bits as per-entry gathers (xstage = 0), and xstage = 1 (the default:
-    only over locally clustered columns) as well; all within the oracle
-    tolerance."""
+    bits as per-entry gathers (xstage = 0) and the default (1); all within
+    the oracle tolerance."""
     rng = np.random.default_rng(17)
     for name, A in _staging_cases(rng):
         A = A.tocsr()
