@@ -69,8 +69,10 @@ static void launch_fwd_vw(int64_t cap, const FwdArgs &a, hipStream_t s) {
 
 // Pairs of entries per lane (VW = 2) for 4-byte-or-smaller values and
 // 32-bit indices when vals / colind are aligned to a pair. Knob "vec": 0
-// (default) pairs 16-bit values, and 4-byte values in column panels (where
-// it measured faster), 1 never pairs, 2 pairs wherever eligible.
+// (default) pairs in column panels (where it measured faster), 1 never
+// pairs, 2 pairs wherever eligible. With every gather of a lane in flight,
+// single entries beat pairs for 16-bit values too (3-D Laplacian 256^3:
+// BF16 323 vs 358 us, FP16 351 vs 358 us).
 template <int MODE, bool BUFX, typename TV, typename TX, typename TY,
           typename TA, typename TI>
 static void launch_fwd(const Plan &p, FwdArgs a, hipStream_t s) {
@@ -91,7 +93,7 @@ static void launch_fwd(const Plan &p, FwdArgs a, hipStream_t s) {
         reinterpret_cast<uintptr_t>(a.vals) % (2 * sizeof(TV)) == 0 &&
         reinterpret_cast<uintptr_t>(a.colind) % 8 == 0;
     const int vec = tuned_vec();
-    const bool want = vec == 2 || (vec == 0 && (sizeof(TV) == 2 || MODE == kRowMap));
+    const bool want = vec == 2 || (vec == 0 && MODE == kRowMap);
     if (aligned && want) {
       launch_fwd_vw<MODE, BUFX, TV, TX, TY, TA, TI, 2>(cap, a, s);
       return;

@@ -83,10 +83,10 @@ const char *drx_last_error(void);
  *                 bands -- else 16; batches use 16). Fixed per call, and per
  *                 batch when it is created.
  *   "vec"         entries per lane of the streaming loads for <= 4-byte
- *                 values with 32-bit indices: 0 (default) two for 16-bit
- *                 values and for column panels, one otherwise; 1 always
- *                 one; 2 two wherever eligible (pairs need vals and colind
- *                 aligned to a pair of entries; others use one).
+ *                 values with 32-bit indices: 0 (default) two in column
+ *                 panels, one otherwise; 1 always one; 2 two wherever
+ *                 eligible (pairs need vals and colind aligned to a pair of
+ *                 entries; others use one).
  *   "wide"        1 (default): FP32 values with 32-bit indices stream
  *                 2048-entry blocks (16 per lane) of plans whose block_nnz
  *                 is the default 1024; 0: the plan's own blocks.
