@@ -309,16 +309,16 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
         // or its own entry unconditionally (a predicated load made each
         // wave wait for one value at a time); n_s > 0 guarantees vals[0].
         const rsrc_t rp = make_rsrc(perm + b.e0, n_s * sizeof(TI));
-        int64_t pe[T];
+        TI pe[T];
 #pragma unroll
         for (int u = 0; u < T; u++) {
           const uint32_t idx = tid + u * WG;
           c[u] = col32(buf_load<TI, kNT>(rc, idx * sizeof(TI)));
-          pe[u] = static_cast<int64_t>(buf_load<TI, kNT>(rp, idx * sizeof(TI)));
+          pe[u] = buf_load<TI, kNT>(rp, idx * sizeof(TI));
         }
         if (n_s > 0) {
 #pragma unroll
-          for (int u = 0; u < T; u++) v[u] = to_acc<TA>(vals[pe[u]]);
+          for (int u = 0; u < T; u++) v[u] = to_acc<TA>(vals[static_cast<int64_t>(pe[u])]);
 #pragma unroll
           for (int u = 0; u < T; u++)
             v[u] = tid + u * WG < static_cast<uint32_t>(n_s) ? v[u] : TA(0);
@@ -498,8 +498,10 @@ template <int WG, int CAP, int MODE, bool BUFX, typename TV, typename TX,
 __global__ __launch_bounds__(WG)
 // (16-bit values in plain mode take one wave fewer: at the LDS-bound count
 // their pair unpacking spilled registers; 3-D Laplacian BF16 390 -> 371 us.
-// The panel kernel's LDS already holds it at 7 waves, where it does not.)
-__attribute__((amdgpu_waves_per_eu(fwd_waves<WG, CAP, MODE, TA, TI>() - (sizeof(TV) == 2 && MODE == kPlain ? 1 : 0), 8)))
+// The panel kernel's LDS already holds it at 7 waves, where it does not.
+// Permuted values take one wave fewer too: their T unconditional value
+// loads spilled at 8; band +-16 FP32 adjoint 563 -> 471 us.)
+__attribute__((amdgpu_waves_per_eu(fwd_waves<WG, CAP, MODE, TA, TI>() - ((sizeof(TV) == 2 && MODE == kPlain) || MODE == kPerm ? 1 : 0), 8)))
 void k_spmv_fwd(FwdArgs a) {
   __shared__ FwdSmem<WG, CAP, MODE == kRowMap, TA, TI> sm;
   for (int64_t wi = blockIdx.x; wi < a.per_xcd * kXcds; wi += a.grid) {
