@@ -120,6 +120,33 @@ def headline(steps, warmup):
     return A, x, y, B, wall_ms, ev_ms, layout, gather
 
 
+def host_cpu_info():
+    """(ranks to use, description): one single-threaded rank per core of the
+    CPU share this process may run on, capped at the 16 cores a one-GPU job
+    is granted on the GPU host (more threads than the share would time
+    oversubscription, not the reference's loop)."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    P = max(1, min(16, affinity))
+    return P, {"nproc": os.cpu_count(), "affinity_cores": affinity, "cpu_model": model,
+               "ranks": P,
+               "launcher": "P pinned std::thread ranks in one process, shared-memory "
+                           "Gatherv (the reference's MPI path cannot be built: no "
+                           "range-v3, mdspan or MPI-4 _c calls in this image)",
+               "rank_cap": "16 = the CPU share of a one-GPU job on the GPU host"}
+
+
 def cpu_baseline(A, x, y_gpu, target_s=10.0):
     """Restated GemvEq_DR CPU path on the first rows of the headline matrix."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
@@ -135,11 +162,7 @@ def cpu_baseline(A, x, y_gpu, target_s=10.0):
     ci = A.colind[:nnz_s].cpu().numpy()
     v = A.vals[:nnz_s].cpu().numpy()
     xs = x.cpu().numpy()
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    P = max(1, min(16, cores))
+    P, host = host_cpu_info()
     # one untimed probe to size the repetitions
     _, t1, _ = orc.gemveq_cpu_bench(rp, ci, v, xs, (R, n), P, warmup=0, reps=1)
     reps = int(max(3, min(50, target_s / max(t1, 1e-3))))
@@ -156,6 +179,7 @@ def cpu_baseline(A, x, y_gpu, target_s=10.0):
                    % (R, nnz_s, reps, P, med)),
         "ms_per_call": round(med * 1e3, 3),
         "gpu_vs_reference_cpu_rel_err_norm2": rel,
+        "host": host,
     }
 
 
@@ -374,11 +398,7 @@ def form_gemveq_band(steps, cpu_seconds=10.0, with_cpu=True):
         orc = _oracle.Oracle()
         rp, ci, v = A.rowptr.cpu().numpy(), A.colind.cpu().numpy(), A.vals.cpu().numpy()
         xs = x.cpu().numpy()
-        try:
-            cores = len(os.sched_getaffinity(0))
-        except AttributeError:
-            cores = os.cpu_count() or 1
-        P = max(1, min(16, cores))
+        P, _ = host_cpu_info()
         _, t1, _ = orc.gemveq_cpu_bench(rp, ci, v, xs, (n, n), P, warmup=0, reps=1)
         reps = int(max(3, min(30, cpu_seconds / max(t1, 1e-3))))
         res, med, _ = orc.gemveq_cpu_bench(rp, ci, v, xs, (n, n), P, warmup=1, reps=reps)

@@ -557,7 +557,10 @@ int drx_batch_create(drx_batch *out, int64_t nitems, const drx_batch_item *items
       int wg;
       pick_kernel(*b, &fn, &wg);  // validates the type combination
       if (b->mrhs) make_chunks(*b, s);
-      const int64_t bn = tuned_block_nnz() > 0 ? tuned_block_nnz() : 1024;
+      // The batch kernels take windows of at most 4096 entries (blocks of
+      // at most 2048, as drx_batch_set_tuning enforces); a larger tuned
+      // default is clamped instead of streaming only part of a window.
+      const int64_t bn = std::min<int64_t>(tuned_block_nnz() > 0 ? tuned_block_nnz() : 1024, 2048);
       replan(*b, bn, s);
     } catch (...) {
       drop_graph(*b);
@@ -655,6 +658,16 @@ int drx_batch_autotune(drx_batch batch, int iters, char *json, int64_t json_cap,
     Batch &b = *reinterpret_cast<Batch *>(batch);
     hipStream_t s = static_cast<hipStream_t>(stream);
     iters = std::max(1, iters);
+    // Timing runs write the items' outputs; with beta forced to 0 they hold
+    // alpha * A x (never a repeatedly accumulated beta * y) afterwards. The
+    // caller's beta is restored for the final plan.
+    const double beta0 = b.beta;
+    struct Restore {
+      Batch &b;
+      double beta;
+      ~Restore() { b.beta = beta; }
+    } restore{b, beta0};
+    b.beta = 0;
     double best = 1e30;
     int64_t best_bn = b.block_nnz;
     int best_g = 0, best_w = b.wg_per_cu;
@@ -675,6 +688,7 @@ int drx_batch_autotune(drx_batch batch, int iters, char *json, int64_t json_cap,
         }
       }
     }
+    b.beta = beta0;
     replan(b, best_bn, s);
     set_gshift(b, best_g, s);
     b.wg_per_cu = best_w;

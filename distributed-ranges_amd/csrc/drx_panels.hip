@@ -236,8 +236,15 @@ static void build_panels(Plan &p, PanelSet &ps, const TI *colind,
     // pair-aligned (two entries per lane per load); a panel starting at an
     // odd entry then has one leading entry that belongs to none of its rows.
     const TI *segp = static_cast<const TI *>(ps.seg_ptr);
+    // A panel without entries gets no plan (the product skips it): with an
+    // odd start its rounded-down base would give it one stray entry and no
+    // rows.
     for (int64_t q = 0; q < ps.npanels; q++) {
       const int64_t rows = ps.pseg[q + 1] - ps.pseg[q];
+      if (ps.pstart[q + 1] == ps.pstart[q]) {
+        ps.plans.push_back(nullptr);
+        continue;
+      }
       const int64_t base = ps.pstart[q] & ~int64_t(1);
       const int64_t pn = ps.pstart[q + 1] - base;
       ps.plans.push_back(make_plan(p.itype, segp + ps.pseg[q], base, rows, p.ncols, pn, 1,

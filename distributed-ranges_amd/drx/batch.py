@@ -6,7 +6,7 @@ import json
 import os
 
 from ._lib import BatchItem, check, lib
-from .core import DTYPE_CODE, CsrMatrix, _cols, _p, _s
+from .core import DTYPE_CODE, CsrMatrix, _cols, _on, _p, _s
 
 
 class Batch:
@@ -26,16 +26,20 @@ class Batch:
         keep = []
         ref = None
         for i, (A, x, y) in enumerate(forward):
-            _, ldx = _cols(x, A.shape[1])
-            _, ldy = _cols(y, A.shape[0])
+            kx, ldx = _cols(_on(x, A.device, "x"), A.shape[1])
+            ky, ldy = _cols(_on(y, A.device, "y"), A.shape[0])
+            if kx != k or ky != k:
+                raise ValueError("batch item operands must hold k = %d right-hand sides" % k)
             items[i] = BatchItem(A.plan.handle.value, None, A.colind.data_ptr(),
                                  A.vals.data_ptr(), x.data_ptr(), ldx, y.data_ptr(), ldy)
             keep += [A, x, y]
             ref = ref or (A.vals.dtype, x.dtype, y.dtype)
         for j, (A, dy, dx) in enumerate(adjoint):
             t = A.transpose(stream)
-            _, lddy = _cols(dy, A.shape[0])
-            _, lddx = _cols(dx, A.shape[1])
+            ky, lddy = _cols(_on(dy, A.device, "dy"), A.shape[0])
+            kx, lddx = _cols(_on(dx, A.device, "dx"), A.shape[1])
+            if kx != k or ky != k:
+                raise ValueError("batch item operands must hold k = %d right-hand sides" % k)
             vptr = None if A.stored_transposed_values(A.vals) else A.vals.data_ptr()
             items[len(forward) + j] = BatchItem(None, t.handle.value, None, vptr,
                                                 dy.data_ptr(), lddy, dx.data_ptr(), lddx)

@@ -122,10 +122,15 @@ class CsrMatrix:
         self.rowptr, self.colind, self.vals = rowptr, colind, vals
         self.shape = (int(shape[0]), int(shape[1]))
         self.nnz = int(colind.numel())
+        self.device = rowptr.device
         if colind.dtype != rowptr.dtype:
             raise ValueError("rowptr and colind must share an index dtype")
-        if vals.numel() != self.nnz:
-            raise ValueError("vals and colind differ in length")
+        if colind.device != self.device or (colind.dim() != 1 or
+                                             (self.nnz > 1 and colind.stride(0) != 1)):
+            raise ValueError("colind must be a contiguous 1-D tensor on rowptr's device")
+        if not rowptr.is_contiguous():
+            raise ValueError("rowptr must be contiguous")
+        _vals_ok(vals, self.nnz, self.device)
         self.base = int(base)
         self.has_sentinel = bool(has_sentinel)
         self.plan = Plan(rowptr, self.shape[0], self.shape[1], self.nnz,
@@ -142,13 +147,15 @@ class CsrMatrix:
     def matvec(self, x, y=None, alpha=1.0, beta=0.0, vals=None, stream=None):
         """y = alpha*A*x + beta*y; x is (n,) or column-major (n, k) given as
         a (k, n) contiguous tensor (row j = right-hand side j)."""
-        vals = self.vals if vals is None else vals
-        k, ldx = _cols(x, self.shape[1])
+        vals = self.vals if vals is None else _vals_ok(vals, self.nnz, self.device)
+        k, ldx = _cols(_on(x, self.device, "x"), self.shape[1])
         ydt = torch.float64 if vals.dtype == torch.float64 else torch.float32
         if y is None:
             shape = (self.shape[0],) if x.dim() == 1 else (k, self.shape[0])
             y = torch.empty(shape, dtype=ydt, device=x.device)
-        _, ldy = _cols(y, self.shape[0])
+        ky, ldy = _cols(_on(y, self.device, "y"), self.shape[0])
+        if ky != k:
+            raise ValueError("x and y differ in their number of right-hand sides")
         check(lib().drx_spmv(self.plan.handle, DTYPE_CODE[vals.dtype],
                              _p(self.colind), _p(vals), DTYPE_CODE[x.dtype],
                              _p(x), ldx, DTYPE_CODE[y.dtype], _p(y), ldy, k,
@@ -170,6 +177,7 @@ class CsrMatrix:
 
     def update_values(self, vals, stream=None):
         """New values for a plan with a panel layout (re-packed in place)."""
+        _vals_ok(vals, self.nnz, self.device)
         check(lib().drx_plan_update_values(self.plan.handle, _p(vals), _s(stream)))
         self.vals = vals
 
@@ -182,7 +190,7 @@ class CsrMatrix:
         """Store `vals` (default: the matrix's) in CSC order in the transpose
         (drx_transpose_set_values): later adjoints with the same, unchanged
         tensor stream them instead of gathering through the permutation."""
-        vals = self.vals if vals is None else vals
+        vals = self.vals if vals is None else _vals_ok(vals, self.nnz, self.device)
         t = self.transpose(stream)
         check(lib().drx_transpose_set_values(t.handle, DTYPE_CODE[vals.dtype], _p(vals),
                                              _s(stream)))
@@ -196,14 +204,16 @@ class CsrMatrix:
 
     def rmatvec(self, dy, dx=None, alpha=1.0, beta=0.0, vals=None, stream=None):
         """dx = alpha*A^T*dy + beta*dx."""
-        vals = self.vals if vals is None else vals
-        t = self.transpose(stream)
-        k, lddy = _cols(dy, self.shape[0])
+        vals = self.vals if vals is None else _vals_ok(vals, self.nnz, self.device)
+        k, lddy = _cols(_on(dy, self.device, "dy"), self.shape[0])
         xdt = torch.float64 if vals.dtype == torch.float64 else torch.float32
         if dx is None:
             shape = (self.shape[1],) if dy.dim() == 1 else (k, self.shape[1])
             dx = torch.empty(shape, dtype=xdt, device=dy.device)
-        _, lddx = _cols(dx, self.shape[1])
+        kx, lddx = _cols(_on(dx, self.device, "dx"), self.shape[1])
+        if kx != k:
+            raise ValueError("dy and dx differ in their number of right-hand sides")
+        t = self.transpose(stream)
         vptr = ctypes.c_void_p(0) if self.stored_transposed_values(vals) else _p(vals)
         check(lib().drx_spmv_adjoint(t.handle, DTYPE_CODE[vals.dtype], vptr,
                                      DTYPE_CODE[dy.dtype], _p(dy), lddy,
@@ -213,13 +223,14 @@ class CsrMatrix:
 
     def dvals(self, x, dy, out=None, alpha=1.0, beta=0.0, stream=None):
         """out[e] = alpha * sum_j dy[row(e), j] x[col(e), j] + beta*out[e]."""
-        k, ldx = _cols(x, self.shape[1])
-        k2, lddy = _cols(dy, self.shape[0])
+        k, ldx = _cols(_on(x, self.device, "x"), self.shape[1])
+        k2, lddy = _cols(_on(dy, self.device, "dy"), self.shape[0])
         if k != k2:
             raise ValueError("x and dy differ in k")
         odt = torch.float64 if x.dtype == torch.float64 else torch.float32
         if out is None:
             out = torch.empty(self.nnz, dtype=odt, device=x.device)
+        _vals_ok(out, self.nnz, self.device)
         check(lib().drx_csr_dvals(self.plan.handle, _p(self.colind),
                                   DTYPE_CODE[x.dtype], _p(x), ldx,
                                   DTYPE_CODE[dy.dtype], _p(dy), lddy, k,
@@ -231,10 +242,19 @@ class CsrMatrix:
         """y = A x, r = y - t, dvals[e] = r[row(e)] * x[col(e)] in one pass."""
         ydt = torch.float64 if self.vals.dtype == torch.float64 else torch.float32
         m = self.shape[0]
+        if x.dim() != 1 or t.dim() != 1:
+            raise ValueError("the fused residual takes one right-hand side (1-D x and t)")
+        _cols(_on(x, self.device, "x"), self.shape[1])
+        _cols(_on(t, self.device, "t"), m)
         y = torch.empty(m, dtype=ydt, device=x.device) if y is None else y
         r = torch.empty(m, dtype=ydt, device=x.device) if r is None else r
         if dvals is None:
             dvals = torch.empty(self.nnz, dtype=ydt, device=x.device)
+        for name, v in (("y", y), ("r", r)):
+            _cols(_on(v, self.device, name), m)
+            if v.dim() != 1:
+                raise ValueError("%s must be 1-D" % name)
+        _vals_ok(dvals, self.nnz, self.device)
         check(lib().drx_spmv_fused_residual(
             self.plan.handle, self.vcode, _p(self.colind), _p(self.vals),
             DTYPE_CODE[x.dtype], _p(x), DTYPE_CODE[ydt], _p(t), _p(y), _p(r),
@@ -249,16 +269,45 @@ class CsrMatrix:
 
 
 def _cols(v, n):
-    """(k, leading dimension) of a (n,) or (k, n) operand."""
+    """(k, leading dimension) of a (n,) or (k, n) operand. The kernels read a
+    1-D operand as n consecutive elements and a 2-D one as k rows of n
+    consecutive elements, so other strides are refused."""
     if v.dim() == 1:
         if v.numel() < n:
             raise ValueError("operand shorter than the matrix dimension")
+        if v.numel() > 1 and v.stride(0) != 1:
+            raise ValueError("1-D operands must have unit stride (got a strided view; "
+                             "pass .contiguous())")
         return 1, max(n, 1)
     if v.dim() == 2:
-        if v.shape[1] < n or v.stride(1) != 1:
+        if v.shape[1] < n or (v.shape[1] > 1 and v.stride(1) != 1):
             raise ValueError("2-D operands are (k, n) with unit column stride")
-        return v.shape[0], v.stride(0)
+        if v.shape[0] > 1 and v.stride(0) < n:
+            raise ValueError("2-D operands need a row stride of at least n")
+        return v.shape[0], max(v.stride(0), n, 1)
     raise ValueError("operand must be 1-D or 2-D")
+
+
+def _on(t, device, name):
+    """Refuse an operand that is not a tensor on the matrix's device (a host
+    pointer handed to a kernel would fault the GPU)."""
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("%s must be a torch.Tensor" % name)
+    if t.device != device:
+        raise ValueError("%s is on %s; the matrix is on %s" % (name, t.device, device))
+    if t.dtype not in DTYPE_CODE:
+        raise ValueError("%s has unsupported dtype %s" % (name, t.dtype))
+    return t
+
+
+def _vals_ok(vals, nnz, device):
+    _on(vals, device, "vals")
+    if vals.dim() != 1 or vals.numel() != nnz:
+        raise ValueError("vals must be 1-D with nnz = %d elements (got shape %s)"
+                         % (nnz, tuple(vals.shape)))
+    if nnz > 1 and vals.stride(0) != 1:
+        raise ValueError("vals must be contiguous")
+    return vals
 
 
 def from_host(rowptr, colind, vals, shape, device="cuda", itype=None,

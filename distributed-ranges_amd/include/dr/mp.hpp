@@ -3,6 +3,7 @@
 // CSR matrix with csr_row / csr_eq distributions, and dr::mp::gemv.
 #pragma once
 
+#include "dr/concepts.hpp"
 #include "dr/detail/generate_csr.hpp"
 #include "dr/detail/index.hpp"
 #include "dr/detail/matrix_io.hpp"

@@ -229,6 +229,10 @@ int drx_spmv_fused_residual(drx_plan plan, int vtype, const void *colind,
  * (instantiated once; drx_batch_graph_launch replays it on any stream).
  * drx_batch_autotune times every tuning combination on the batch and keeps
  * the fastest (invalidates a captured graph); its JSON describes the choice.
+ * It runs the items about a hundred times with beta taken as 0, so on return
+ * every output holds alpha * (product), whatever it held before: with
+ * beta != 0, restore the outputs and run again. A default block size from
+ * drx_tune_set("block_nnz") above 2048 is clamped to 2048 for batches.
  * No counterpart in the reference: it batches the per-share gemv seam. */
 typedef struct drx_batch_item {
   drx_plan plan;           /* forward item: the matrix's plan (else NULL) */

@@ -4,6 +4,9 @@
 // GemvEq_DR input (band matrix, width 8, x[i + j*n] = i*j + 1) checked for
 // exact equality against a host loop (examples/mp/sparse_matrix.cpp:84-102).
 #include <cstdlib>
+#include <iterator>
+#include <ranges>
+#include <utility>
 #include <vector>
 
 #include "dr/mp.hpp"
@@ -95,6 +98,36 @@ TEST(SparseMatrix, IterationMatchesInput) {
   }
   dr::destroy_csr_matrix_view(csr);
 }
+
+// The distributions satisfy the plug-in concepts, and the matrix is a
+// forward range with segments (test/gtest/mp/sparse_matrix.cpp:34-70).
+template <template <typename, typename, typename> class Distr> void static_assert_case() {
+  using Dist = Distr<float, unsigned long, mp::MpiBackend>;
+  static_assert(mp::matrix_distibution<Dist>);
+  static_assert(mp::vector_multiplicable<Dist>);
+  using Matrix = mp::distributed_sparse_matrix<float, unsigned long, mp::MpiBackend, Dist>;
+  static_assert(std::forward_iterator<decltype(std::declval<Matrix &>().begin())>);
+  static_assert(std::forward_iterator<decltype(std::declval<Matrix &>().end())>);
+  static_assert(std::forward_iterator<decltype(std::ranges::begin(std::declval<Matrix &>()))>);
+  static_assert(std::forward_iterator<decltype(std::ranges::end(std::declval<Matrix &>()))>);
+  static_assert(std::ranges::forward_range<Matrix>);
+  static_assert(dr::distributed_range<Matrix>);
+  // the default distribution of the reference's test (csr_eq) as well
+  static_assert(dr::distributed_range<mp::distributed_sparse_matrix<float, unsigned long>>);
+  // and a constructed matrix iterates as one
+  auto csr = dr::generate_random_csr<float, unsigned long>(dr::index<unsigned long>(100, 100), 0.1f);
+  mp::run_ranks(2, [&] {
+    Matrix a(csr, 0);
+    std::size_t n = 0;
+    for (auto it = std::ranges::begin(a); it != std::ranges::end(a); ++it) n++;
+    EXPECT_EQ(n, csr.size());
+    EXPECT_EQ(dr::ranges::segments(a).size(), std::size_t(2));
+  });
+  dr::destroy_csr_matrix_view(csr);
+}
+
+TEST(SparseMatrix, staticAssertEq) { static_assert_case<mp::csr_eq_distribution>(); }
+TEST(SparseMatrix, staticAssertRow) { static_assert_case<mp::csr_row_distribution>(); }
 
 // GemvEq_DR's input at the mp-bench ctest size (n = 866, band +-17, f64/long,
 // width 8, x[i + j*n] = i*j + 1): all-ones band, so every product is an

@@ -96,3 +96,49 @@ def test_gemm_argument_validation_without_gpu():
     assert L.drx_tune_get(b"xstage", ctypes.byref(v)) == 0 and v.value == 1
     assert L.drx_tune_set(b"xstage", 3) == bad
     assert L.drx_tune_set(b"xstage", 2) == 0 and L.drx_tune_set(b"xstage", 1) == 0
+
+
+def test_shim_refuses_bad_operands_without_gpu():
+    """The ctypes shim refuses operands the kernels would misread: tensors on
+    another device (a host pointer would fault the GPU), strided 1-D views
+    (read as if contiguous), values of the wrong length, mismatched k. All
+    refusals happen before any library call (meta tensors stand in for the
+    matrix's device)."""
+    import torch
+    from drx.core import CsrMatrix
+    dev = torch.device("meta")
+    A = CsrMatrix.__new__(CsrMatrix)
+    A.shape, A.nnz, A.device = (4, 5), 3, dev
+    A.vals = torch.empty(3, device=dev)
+    A.colind = torch.empty(3, dtype=torch.int32, device=dev)
+    A._t, A._tvals_key = None, None
+    x = torch.empty(5, device=dev)
+    with pytest.raises(ValueError, match="is on cpu"):
+        A.matvec(torch.ones(5))
+    with pytest.raises(ValueError, match="y is on cpu"):
+        A.matvec(x, y=torch.ones(4))
+    X = torch.empty(5, 2, device=dev)
+    with pytest.raises(ValueError, match="unit stride"):
+        A.matvec(X[:, 0])
+    with pytest.raises(ValueError, match="nnz"):
+        A.matvec(x, vals=torch.empty(4, device=dev))
+    with pytest.raises(ValueError, match="contiguous"):
+        A.matvec(x, vals=torch.empty(6, device=dev)[::2])
+    with pytest.raises(ValueError, match="right-hand sides"):
+        A.matvec(torch.empty(2, 5, device=dev), y=torch.empty(3, 4, device=dev))
+    with pytest.raises(ValueError, match="unit column stride"):
+        A.matvec(torch.empty(5, 2, device=dev).t())
+    with pytest.raises(ValueError, match="dy is on cpu"):
+        A.rmatvec(torch.ones(4))
+    with pytest.raises(ValueError, match="unit stride"):
+        A.rmatvec(torch.empty(4, 2, device=dev)[:, 1])
+    with pytest.raises(ValueError, match="x is on cpu"):
+        A.dvals(torch.ones(5), torch.empty(4, device=dev))
+    with pytest.raises(ValueError, match="nnz"):
+        A.dvals(x, torch.empty(4, device=dev), out=torch.empty(2, device=dev))
+    with pytest.raises(ValueError, match="t is on cpu"):
+        A.fused_residual(x, torch.ones(4))
+    with pytest.raises(ValueError, match="nnz"):
+        A.update_values(torch.empty(2, device=dev))
+    with pytest.raises(ValueError, match="unsupported dtype"):
+        A.matvec(torch.empty(5, dtype=torch.int32, device=dev))
