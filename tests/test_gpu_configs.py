@@ -15,8 +15,11 @@ Tolerances are componentwise, |y - y*| / (|A||x|), against the oracle
 compare with fp_equal at 128 eps, test/gtest/include/common-tests.hpp:82-95;
 componentwise 1e-5 is stricter for long rows). 16-bit storage: 1e-5 against
 the oracle on the dequantized values and x (accumulation error), and the
-storage rounding bound (2 u + u^2 with u = 2^-8 for BF16, 2^-11 for FP16,
-plus the accumulation tolerance) against the oracle on the FP32 inputs.
+storage rounding bound against the oracle on the FP32 inputs: per row,
+(2 u + u^2) |A||x| plus eta ((1 + u)(|v| + |x|) + eta) summed over the row's entries
+(u = 2^-8 for BF16, 2^-11 for FP16; eta = half the subnormal spacing, which
+bounds the rounding of values below the smallest normal), plus the
+accumulation tolerance.
 """
 import numpy as np
 import pytest
@@ -29,6 +32,9 @@ pytestmark = pytest.mark.gpu
 
 DEV = "cuda"
 U16 = {torch.bfloat16: 2.0 ** -8, torch.float16: 2.0 ** -11}
+# Half the spacing of the subnormals: the absolute rounding error of values
+# below the smallest normal (FP16: 6.1e-5; BF16 shares FP32's exponent range).
+ETA16 = {torch.bfloat16: 2.0 ** -134, torch.float16: 2.0 ** -25}
 
 
 def _host(A, vals=None):
@@ -77,9 +83,16 @@ def test_zipf_16bit_storage_column_panels(orc, vdt):
     # total error against the FP32 inputs: bounded by the storage rounding
     v32 = A.vals.cpu().numpy()
     yo, abso = orc.spmv(rp, ci, v32, x32.cpu().numpy())
-    u = U16[vdt]
-    assert componentwise_err(y16.cpu().numpy(), yo, abso) < 2 * u + u * u + 1e-5
-    assert componentwise_err(y32x.cpu().numpy(), yo, abso) < u + 1e-5
+    u, eta = U16[vdt], ETA16[vdt]
+    xa = np.abs(x32.double().cpu().numpy())[ci]
+    va = np.abs(v32.astype(np.float64))
+    # storage rounding of both operands, and of the values only (FP32 x)
+    sub_both = np.add.reduceat(eta * ((1 + u) * (xa + va) + eta), rp[:-1])
+    sub_vals = np.add.reduceat(eta * xa, rp[:-1])
+    d16 = np.abs(y16.cpu().numpy().astype(np.float64) - yo)
+    d32 = np.abs(y32x.cpu().numpy().astype(np.float64) - yo)
+    assert bool((d16 <= (2 * u + u * u + 1e-5) * abso + sub_both).all())
+    assert bool((d32 <= (u + 1e-5) * abso + sub_vals).all())
     # the panel plan and the plain plan agree within the accumulation bound
     plain = drx.CsrMatrix(A.rowptr, A.colind, vq, A.shape)
     yp = plain.matvec(xq)
