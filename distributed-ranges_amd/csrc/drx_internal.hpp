@@ -44,6 +44,9 @@ struct Plan {
   // k >= 2 path once.
   mutable std::atomic<const void *> loc_colind{nullptr};
   mutable std::atomic<int> loc_local{-1};
+  // Records in a caller's buffer (init_plan_in): products take no scratch
+  // (k >= 2 gathers the k columns of x) and probe nothing.
+  bool no_scratch = false;
 };
 
 // Launch shape over a plan's blocks: grid workgroups walk per_xcd * 8
@@ -111,6 +114,15 @@ Plan *make_plan(int itype, const void *rowptr, int64_t base, int64_t nrows,
                 int64_t ncols, int64_t nnz, int sent, int64_t block_nnz,
                 hipStream_t s, bool validate, bool async_alloc);
 void free_plan(Plan *p, hipStream_t s);
+// Records a share of nnz entries needs with block_nnz-entry blocks (plus the
+// 2048-entry set of 1024-entry plans), sentinels included.
+int64_t records_needed(int64_t nnz, int64_t block_nnz);
+// A plan of 1024-entry blocks whose records live in `buf` (buf_records
+// records, device memory the caller owns); allocates nothing, launches the
+// record kernels on s. Throws if the buffer is too small.
+void init_plan_in(Plan &p, int itype, const void *rowptr, int64_t base, int64_t nrows,
+                  int64_t ncols, int64_t nnz, int sent, BlockRec *buf,
+                  int64_t buf_records, hipStream_t s);
 
 void run_spmv(const Plan &p, int vtype, const void *colind, const void *vals,
               int xtype, const void *x, int64_t ldx, int ytype, void *y,

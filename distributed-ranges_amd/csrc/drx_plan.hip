@@ -72,13 +72,15 @@ int64_t default_block_nnz(int itype) {
   return 1024;
 }
 
-// Records of blocks of `nb` entries (window 2 nb) into a new array.
+// Records of blocks of `nb` entries (window 2 nb) into a new array, or into
+// `into` (caller-owned, at least nblocks + 1 records) when it is given.
 static BlockRec *make_records(const Plan &p, int64_t nb, int64_t *nblocks, hipStream_t s,
-                              bool async_alloc) {
+                              bool async_alloc, BlockRec *into = nullptr) {
   *nblocks = p.nnz / nb + 1;
   const size_t bytes = static_cast<size_t>(*nblocks + 1) * sizeof(BlockRec);
-  BlockRec *rec = nullptr;
-  if (async_alloc)
+  BlockRec *rec = into;
+  if (rec != nullptr) {
+  } else if (async_alloc)
     DRX_HIP(hipMallocAsync(reinterpret_cast<void **>(&rec), bytes, s));
   else
     DRX_HIP(hipMalloc(reinterpret_cast<void **>(&rec), bytes));
@@ -106,6 +108,12 @@ static void build_records(Plan &p, hipStream_t s, bool async_alloc) {
   p.rec = make_records(p, p.block_nnz, &p.nblocks, s, async_alloc);
   if (p.block_nnz == 1024)
     p.rec_wide = make_records(p, 2048, &p.nblocks_wide, s, async_alloc);
+}
+
+int64_t records_needed(int64_t nnz, int64_t block_nnz) {
+  int64_t n = nnz / block_nnz + 2;
+  if (block_nnz == 1024) n += nnz / 2048 + 2;
+  return n;
 }
 
 static void validate_plan_args(int itype, const void *rowptr, int64_t nrows,
@@ -172,6 +180,27 @@ Plan *make_plan(int itype, const void *rowptr, int64_t base, int64_t nrows,
     throw;
   }
   return p;
+}
+
+void init_plan_in(Plan &p, int itype, const void *rowptr, int64_t base, int64_t nrows,
+                  int64_t ncols, int64_t nnz, int sent, BlockRec *buf,
+                  int64_t buf_records, hipStream_t s) {
+  validate_plan_args(itype, rowptr, nrows, ncols, nnz, 0);
+  p.itype = itype;
+  p.rowptr = rowptr;
+  p.base = base;
+  p.nrows = nrows;
+  p.ncols = ncols;
+  p.nnz = nnz;
+  p.has_sentinel = sent ? 1 : 0;
+  p.block_nnz = 1024;
+  p.cap = 2 * p.block_nnz;
+  p.no_scratch = true;
+  DRX_REQUIRE(records_needed(nnz, p.block_nnz) <= buf_records,
+              "the share needs more plan records than the buffer holds");
+  DRX_HIP(hipGetDevice(&p.device));
+  p.rec = make_records(p, p.block_nnz, &p.nblocks, s, false, buf);
+  p.rec_wide = make_records(p, 2048, &p.nblocks_wide, s, false, buf + p.nblocks + 1);
 }
 
 void free_plan(Plan *p, hipStream_t s) {

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "drx_common.hpp"
@@ -186,7 +187,7 @@ static void run_spmv_impl(const Plan &p, int vtype, const void *colind,
     // (2-D Laplacian FP64, k = 2 / 3 / 4: 3009 / 2891 / 2811 GB/s through
     // column gathers vs 2151 / 2245 / 2572 through chunks; a +-16 band, 33
     // entries per row, is faster through chunks at every k).
-    const int mr = tuned_mrhs();
+    const int mr = p.no_scratch ? 0 : tuned_mrhs();
     const bool stencil = k <= 4 && p.nnz <= 8 * p.nrows;
     if (k >= 2 && mr != 0 &&
         (mr == 2 || !stencil || (column_locality(p, colind, s) & 1) == 0)) {
@@ -252,11 +253,74 @@ void run_spmv_perm(const Plan &p, int vtype, const void *colind,
                        ytype, y, ldy, k, alpha, beta, s);
 }
 
+// ---------------------------------------------------------------------------
+// Scratch for the entry point that allocates nothing (drx_spmv_csr_noalloc):
+// the plan records of one call live in this file-scope device array, one
+// copy per GPU, held until the process ends. Its device address comes from
+// hipGetSymbolAddress once per device (with that device current); host code
+// passes that address, never the array's name, to kernels.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoallocRecords = 262144;  // 8 MiB of 32-byte records
+static __device__ __align__(256) BlockRec scratch[kNoallocRecords];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static BlockRec *g_scratch_ptr[kMaxDevices] = {};
+
+static BlockRec *scratch_for_device(int dev) {
+  DRX_REQUIRE(dev >= 0 && dev < kMaxDevices, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void *ptr = nullptr;
+    DRX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    DRX_REQUIRE(ptr != nullptr, "no device address for the scratch records");
+    g_scratch_ptr[dev] = static_cast<BlockRec *>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+
 }  // namespace drx
 
 using namespace drx;
 
 extern "C" {
+
+int64_t drx_spmv_csr_noalloc_max_nnz(void) {
+  // largest nnz with records_needed(nnz, 1024) <= kNoallocRecords
+  int64_t lo = 0, hi = kNoallocRecords * int64_t(1024);
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo + 1) / 2;
+    if (records_needed(mid, 1024) <= kNoallocRecords)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+int drx_spmv_csr_noalloc(int itype, const void *rowptr, int64_t rowptr_base,
+                         int64_t nrows, int64_t ncols, int64_t nnz,
+                         int has_sentinel, const void *colind, int vtype,
+                         const void *vals, int xtype, const void *x,
+                         int64_t ldx, int ytype, void *y, int64_t ldy,
+                         int64_t k, double alpha, double beta,
+                         drx_stream_t stream) {
+  return guarded([&] {
+    DRX_REQUIRE(nnz >= 0 && records_needed(nnz, 1024) <= kNoallocRecords,
+                "drx_spmv_csr_noalloc: the share exceeds the scratch records "
+                "(see drx_spmv_csr_noalloc_max_nnz)");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int dev = 0;
+    DRX_HIP(hipGetDevice(&dev));
+    BlockRec *buf = scratch_for_device(dev);
+    Plan p;
+    init_plan_in(p, itype, rowptr, rowptr_base, nrows, ncols, nnz, has_sentinel,
+                 buf, kNoallocRecords, s);
+    run_spmv(p, vtype, colind, vals, xtype, x, ldx, ytype, y, ldy, k, alpha,
+             beta, s);
+  });
+}
+
 
 int drx_spmv(drx_plan plan, int vtype, const void *colind, const void *vals,
              int xtype, const void *x, int64_t ldx, int ytype, void *y,

@@ -52,6 +52,10 @@ SIGNATURES = {
     "drx_spmv_csr": (c_int, [c_int, c_vp, c_i64, c_i64, c_i64, c_i64, c_int,
                              c_vp, c_int, c_vp, c_int, c_vp, c_i64, c_int,
                              c_vp, c_i64, c_i64, c_dbl, c_dbl, c_vp]),
+    "drx_spmv_csr_noalloc": (c_int, [c_int, c_vp, c_i64, c_i64, c_i64, c_i64, c_int,
+                                     c_vp, c_int, c_vp, c_int, c_vp, c_i64, c_int,
+                                     c_vp, c_i64, c_i64, c_dbl, c_dbl, c_vp]),
+    "drx_spmv_csr_noalloc_max_nnz": (c_i64, []),
     "drx_transpose_create": (c_int, [ctypes.POINTER(c_vp), c_vp, c_vp, c_vp]),
     "drx_transpose_destroy": (c_int, [c_vp]),
     "drx_transpose_arrays": (c_int, [c_vp, ctypes.POINTER(c_vp),

@@ -172,6 +172,28 @@ int drx_spmv_csr(int itype, const void *rowptr, int64_t rowptr_base,
                  const void *x, int64_t ldx, int ytype, void *y, int64_t ldy,
                  int64_t k, double alpha, double beta, drx_stream_t stream);
 
+/* The one-shot form above without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). No counterpart in the reference. The
+ * call's plan records live in a library-owned device array of 8 MiB per GPU
+ * (held until the process ends); the caller passes inputs and outputs only.
+ * Shares of more than drx_spmv_csr_noalloc_max_nnz() entries (about 178M)
+ * are rejected with DRX_ERR_INVALID. k >= 2 gathers the k columns of x
+ * directly (no row-major copy of x). Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (different streams or threads without
+ * ordering) race and give wrong results; calls ordered on one stream are
+ * safe. Nothing is cleared between calls (each call writes every record it
+ * reads). Same arguments, semantics and status codes as drx_spmv_csr. */
+int drx_spmv_csr_noalloc(int itype, const void *rowptr, int64_t rowptr_base,
+                         int64_t nrows, int64_t ncols, int64_t nnz,
+                         int has_sentinel, const void *colind, int vtype,
+                         const void *vals, int xtype, const void *x,
+                         int64_t ldx, int ytype, void *y, int64_t ldy,
+                         int64_t k, double alpha, double beta,
+                         drx_stream_t stream);
+/* Largest nnz drx_spmv_csr_noalloc accepts (host arithmetic, no device work). */
+int64_t drx_spmv_csr_noalloc_max_nnz(void);
+
 /* ---- adjoint ------------------------------------------------------------
  * No counterpart in the reference (it has no gradient path); these serve the
  * differentiation of the gemv above.
