@@ -444,6 +444,39 @@ def form_dense_gemm(steps, n=16000):
     return out
 
 
+def stream_ceiling(steps=10):
+    """HBM read ceiling of this box: drx_stream_read (16-byte non-temporal
+    loads, 8 in flight per lane) over a 4 GiB buffer."""
+    buf = torch.empty(1 << 30, dtype=torch.float32, device="cuda")
+    buf.fill_(1.0)
+    digest = torch.empty(1 << 20, dtype=torch.int32, device="cuda")
+    _, ms = time_launches(lambda: drx.gen.stream_read(buf, digest), steps, 2)
+    nbytes = buf.numel() * 4
+    del buf, digest
+    torch.cuda.empty_cache()
+    return {"GBps": round(nbytes / ms / 1e6, 1), "frac_of_peak": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4),
+            "probe": "drx_stream_read over 4 GiB (16-byte non-temporal loads), mean of %d launches"
+                     % steps}
+
+
+def form_mp_host(reps=20):
+    """The reference benchmark's call shape, gemv(0, res, m, b) on a csr_eq
+    distributed_sparse_matrix (benchmarks/gbench/mp/gemv.cpp:41-64), timed
+    through the C++ host interface on the headline matrix
+    (distributed-ranges_amd/bin/mp_gemv_bench, one rank)."""
+    import subprocess
+    exe = os.path.join(REPO, "distributed-ranges_amd", "bin", "mp_gemv_bench")
+    if not os.path.exists(exe):
+        return {"error": "mp_gemv_bench not built"}
+    p = subprocess.run([exe, "16000000", str(reps)], capture_output=True, text=True, timeout=600)
+    line = [ln for ln in p.stdout.splitlines() if ln.startswith("{")]
+    if p.returncode != 0 or not line:
+        return {"error": "rc %d: %s" % (p.returncode, (p.stderr or p.stdout)[-400:])}
+    out = json.loads(line[-1])
+    out["call"] = "dr::mp::gemv(0, res, m, b): share product + partial to host + gather + fence"
+    return out
+
+
 # ---------------------------------------------------------------------------
 def read_traffic():
     """HBM bytes per launch of the headline kernel from the committed
@@ -512,17 +545,19 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": read_traffic(),
                      "kernel": "drx::k_spmv_fwd<256,2048,kRowMap,f32,i32> (one launch per "
-                               "column panel) + k_scale_y; achieved = bytes / summed time of "
-                               "one step's launches",
+                               "column panel); achieved = bytes / summed time of one step's "
+                               "launches",
                      "ms_per_step_kernels": round(ev_ms, 4),
                      "gather_ceiling": gather},
     }
+    log("stream ceiling ...")
+    result["roofline"]["stream_ceiling"] = stream_ceiling()
     if not args.no_cpu:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(A, x, y, args.cpu_seconds)
     forms = {}
     want = set(["laplacian2d_f64", "band_adjoint", "lowprec", "batched", "gemveq_band",
-                "dense_gemm"]) \
+                "dense_gemm", "mp_host"]) \
         if args.forms == "all" else \
         set() if args.forms == "none" else set(args.forms.split(","))
     steps_f = max(5, min(args.steps, 20))
@@ -546,6 +581,10 @@ def main():
     if "dense_gemm" in want:
         log("form dense_gemm ...")
         forms["dense_gemm"] = form_dense_gemm(steps_f)
+    if "mp_host" in want:
+        log("form mp_host ...")
+        torch.cuda.empty_cache()
+        forms["mp_host"] = form_mp_host(steps_f)
     if forms:
         result["forms"] = forms
     print(json.dumps(result), flush=True)

@@ -122,7 +122,7 @@ static void build_transpose(Transpose &t, const Plan &p, const TI *colind,
     DRX_HIP(hipStreamSynchronize(s));
   }
   t.plan = make_plan(p.itype, t.colptr, 0, ncols, p.nrows, nnz, 1, 0, s,
-                     false, false);
+                     false, false, true);
   DRX_HIP(hipStreamSynchronize(s));
 }
 

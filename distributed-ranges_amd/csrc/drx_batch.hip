@@ -391,7 +391,7 @@ static void replan(Batch &b, int64_t block_nnz, hipStream_t s) {
     const BatchItemHost &it = b.items[i];
     const Plan &src = *it.src;
     Plan *p = make_plan(src.itype, src.rowptr, src.base, src.nrows, src.ncols,
-                        src.nnz, src.has_sentinel, block_nnz, s, false, false);
+                        src.nnz, src.has_sentinel, block_nnz, s, false, false, true);
     b.plans.push_back(p);
     FwdArgs a{};
     a.rowptr = p->rowptr;

@@ -221,6 +221,10 @@ __device__ __forceinline__ int64_t row_begin(const TI *rowptr, int64_t base,
   return nnz;
 }
 
+// Top bit of a column panel's row-map entry: the segment is its row's first
+// in panel order (row ids are non-negative, so the bit is free).
+template <typename TI> constexpr TI kFirstSeg = static_cast<TI>(TI(1) << (sizeof(TI) * 8 - 1));
+
 // Number of XCDs that a launch's workgroups are dealt over round-robin.
 constexpr int kXcds = 8;
 

@@ -50,6 +50,21 @@ struct FwdArgs {
   int64_t ncols;   // columns of the share (rows of one xr chunk)
 };
 
+// Row-map entries of column panels carry in their top bit whether the
+// segment is its row's first in panel order. That segment writes
+// y = beta * y + alpha * sum (y not read when beta is 0); the row's later
+// segments add alpha * sum to it.
+template <typename TI> __device__ __forceinline__ int64_t map_row(TI m) {
+  return static_cast<int64_t>(m & static_cast<TI>(~kFirstSeg<TI>));
+}
+// y of a segment's row before its sum is added.
+template <typename TA, typename TY, typename TI>
+__device__ __forceinline__ TA map_ypre(const TY *yj, TI m, TA beta) {
+  const int64_t row = map_row(m);
+  if ((m & kFirstSeg<TI>) == 0) return to_acc<TA>(yj[row]);
+  return beta == TA(0) ? TA(0) : beta * to_acc<TA>(yj[row]);
+}
+
 template <int WG, typename TA>
 __device__ __forceinline__ TA block_sum(TA v, TA *red) {
   v = group_sum<64>(v);
@@ -91,10 +106,10 @@ __device__ __forceinline__ void reduce_rows(const TA *prod, const int *rp_s,
     if (lane == 0) {
       if constexpr (MODE == kRowMap) {
         if (ypre != nullptr && rows_lds) {
-          yj[static_cast<int64_t>(map_s[i])] = from_acc<TY>(ypre[i] + alpha * acc);
+          yj[map_row(map_s[i])] = from_acc<TY>(ypre[i] + alpha * acc);
         } else {
-          const int64_t row = static_cast<int64_t>(rowmap[r0 + i]);
-          yj[row] = from_acc<TY>(to_acc<TA>(yj[row]) + alpha * acc);
+          const TI m = rowmap[r0 + i];
+          yj[map_row(m)] = from_acc<TY>(map_ypre<TA>(yj, m, beta) + alpha * acc);
         }
       } else {
         TA out = alpha * acc;
@@ -246,7 +261,7 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
 #pragma unroll
         for (int k = 0; k < NI; k++) {
           if (k * WG > nr) break;
-          yv[k] = to_acc<TA>(y[static_cast<int64_t>(row[k])]);
+          yv[k] = map_ypre<TA>(y, row[k], beta);
         }
       }
 #pragma unroll
@@ -344,7 +359,7 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
           if (i < nr) {
             const TI row = static_cast<const TI *>(a.rowmap)[b.r0 + i];
             map_s[i] = row;
-            if (a.k == 1) ypre[i] = to_acc<TA>(y[row]);
+            if (a.k == 1) ypre[i] = map_ypre<TA>(y, row, beta);
           }
         }
       }
@@ -396,11 +411,13 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
       if (staged) {
       } else if constexpr (BUFX) {
         const rsrc_t rx = make_rsrc(xj, a.xbytes);
-        if constexpr (sizeof(TA) == 8) {
-          // 8-byte sums: one gather in flight per entry slot. Holding all T
-          // gathered values costs T more register pairs, which spill at 7
-          // waves or cost a wave (3-D Laplacian 256^3 FP64: 478 us this way,
-          // 492-652 us with all gathers in flight).
+        if constexpr (sizeof(TA) == 8 && T > 4) {
+          // 8-byte sums over 8 or more entries per lane: one gather in
+          // flight per entry slot. Holding all T gathered values costs T
+          // more register pairs, which spill at 7 waves or cost a wave (3-D
+          // Laplacian 256^3 FP64: 478 us this way, 492-652 us with all
+          // gathers in flight). Windows of 1024 entries (4 per lane) take
+          // the path below.
 #pragma unroll
           for (int u = 0; u < T; u++) {
             const int idx = eidx(u);
@@ -467,8 +484,9 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
         const TA s = block_sum<WG>(acc0 + acc1, red);
         if (tid == 0) {
           if constexpr (MODE == kRowMap) {
-            const int64_t row = static_cast<int64_t>(static_cast<const TI *>(a.rowmap)[last]);
-            y[row + j * a.ldy] = from_acc<TY>(to_acc<TA>(y[row + j * a.ldy]) + alpha * s);
+            const TI m = static_cast<const TI *>(a.rowmap)[last];
+            TY *yj = y + j * a.ldy;
+            yj[map_row(m)] = from_acc<TY>(map_ypre<TA>(yj, m, beta) + alpha * s);
           } else {
             TA out = alpha * s;
             if (beta != TA(0)) out += beta * to_acc<TA>(y[last + j * a.ldy]);

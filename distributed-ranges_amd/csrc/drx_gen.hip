@@ -36,6 +36,12 @@ static std::atomic<int> g_gemm_tile{0};
 int tuned_gemm_tile() { return g_gemm_tile.load(); }
 static std::atomic<int> g_xstage{1};
 int tuned_xstage() { return g_xstage.load(); }
+static std::atomic<int> g_auto_panels{1};
+int tuned_auto_panels() { return g_auto_panels.load(); }
+static std::atomic<int> g_fill{1};
+int tuned_fill() { return g_fill.load(); }
+static std::atomic<int> g_fwd_cap{0};
+int tuned_fwd_cap() { return g_fwd_cap.load(); }
 
 int64_t tuned_block_nnz() {
   int64_t v = g_block_nnz.load();
@@ -115,6 +121,34 @@ __global__ __launch_bounds__(256) void k_gather(const int32_t *idx, int64_t n,
     for (int u = 0; u < 4; u++)
       if (e[u] < n) out[e[u]] = v[u];
   }
+}
+
+// Read-rate probe: every workgroup streams 16-byte chunks (non-temporal
+// loads, 8 in flight per lane) over its share of [0, n16) and writes one
+// 4-byte digest per workgroup, so that the loads cannot be dropped. The
+// benchmark times it over a buffer far larger than the caches to measure the
+// HBM read ceiling of the box.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_stream_read(const u32x4 *src, int64_t n16,
+                                                     uint32_t *out) {
+  uint32_t acc = 0;
+  const int64_t step = static_cast<int64_t>(gridDim.x) * 256 * 8;
+  for (int64_t base = static_cast<int64_t>(blockIdx.x) * 256 * 8 + threadIdx.x; base < n16;
+       base += step) {
+    u32x4 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int64_t i = base + u * 256;
+      v[u] = i < n16 ? __builtin_nontemporal_load(src + i) : u32x4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+  }
+  for (int o = 32; o > 0; o >>= 1) acc ^= __shfl_xor(acc, o, 64);
+  __shared__ uint32_t red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0] ^ red[1] ^ red[2] ^ red[3];
 }
 
 static dim3 grid1d(int64_t n) {
@@ -360,6 +394,16 @@ int drx_tune_set(const char *key, int64_t value) {
     } else if (std::strcmp(key, "xstage") == 0) {
       DRX_REQUIRE(value >= 0 && value <= 2, "xstage must be 0, 1 or 2");
       g_xstage.store(static_cast<int>(value));
+    } else if (std::strcmp(key, "fill") == 0) {
+      DRX_REQUIRE(value == 0 || value == 1, "fill must be 0 or 1");
+      g_fill.store(static_cast<int>(value));
+    } else if (std::strcmp(key, "fwd_cap") == 0) {
+      DRX_REQUIRE(value == 0 || value == 2048 || value == 4096,
+                  "fwd_cap must be 0, 2048 or 4096");
+      g_fwd_cap.store(static_cast<int>(value));
+    } else if (std::strcmp(key, "auto_panels") == 0) {
+      DRX_REQUIRE(value == 0 || value == 1, "auto_panels must be 0 or 1");
+      g_auto_panels.store(static_cast<int>(value));
     } else if (std::strcmp(key, "gemm_tile") == 0) {
       DRX_REQUIRE(value >= 0 && value <= 7, "gemm_tile must be in [0, 7]");
       g_gemm_tile.store(static_cast<int>(value));
@@ -390,6 +434,12 @@ int drx_tune_get(const char *key, int64_t *value) {
       *value = g_gemm_tile.load();
     else if (std::strcmp(key, "xstage") == 0)
       *value = g_xstage.load();
+    else if (std::strcmp(key, "auto_panels") == 0)
+      *value = g_auto_panels.load();
+    else if (std::strcmp(key, "fill") == 0)
+      *value = g_fill.load();
+    else if (std::strcmp(key, "fwd_cap") == 0)
+      *value = g_fwd_cap.load();
     else
       throw Error(DRX_ERR_INVALID, std::string("unknown tuning key: ") + key);
   });
@@ -629,6 +679,25 @@ int drx_gather(const int32_t *idx, int64_t n, int xtype, const void *x,
       case 4: hipLaunchKernelGGL(k_gather<uint32_t>, g, dim3(256), 0, s, idx, n, x, xb, static_cast<uint32_t *>(out)); break;
       default: hipLaunchKernelGGL(k_gather<uint16_t>, g, dim3(256), 0, s, idx, n, x, xb, static_cast<uint16_t *>(out)); break;
     }
+    DRX_HIP(hipGetLastError());
+  });
+}
+
+int drx_stream_read(const void *src, int64_t bytes, void *digest,
+                    int64_t digest_len, drx_stream_t stream) {
+  return guarded([&] {
+    DRX_REQUIRE(bytes >= 0 && bytes % 16 == 0, "bytes must be a multiple of 16");
+    DRX_REQUIRE(bytes == 0 || (src && digest), "bad argument");
+    DRX_REQUIRE(reinterpret_cast<uintptr_t>(src) % 16 == 0, "src must be 16-byte aligned");
+    DRX_REQUIRE(digest_len >= 1, "digest_len must be at least 1");
+    if (bytes == 0) return;
+    const int64_t n16 = bytes / 16;
+    const int64_t per = 256 * 8;
+    const int64_t g = std::min<int64_t>(std::min<int64_t>((n16 + per - 1) / per, digest_len),
+                                        int64_t(1) << 20);
+    hipLaunchKernelGGL(k_stream_read, dim3(static_cast<unsigned>(g)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), static_cast<const u32x4 *>(src), n16,
+                       static_cast<uint32_t *>(digest));
     DRX_HIP(hipGetLastError());
   });
 }

@@ -33,6 +33,8 @@ struct Plan {
   // the forward kernel's layout for 4-byte values (16 entries per lane).
   BlockRec *rec_wide = nullptr;
   int64_t nblocks_wide = 0;
+  // Longest row of the share (-1: not measured); sets the block pitch.
+  int64_t lmax = -1;
   bool owns_async = false;
   int device = 0;
   PanelSet *panels = nullptr;  // column-panel layout (drx_plan_set_panels)
@@ -89,6 +91,13 @@ int64_t tuned_grid_cap();
 int tuned_gemm_tile();
 // x staging in LDS for k = 1: 0 never, 1 over locally clustered columns, 2 always.
 int tuned_xstage();
+// drx_plan_auto_panels: 1 builds panels when they pay, 0 never.
+int tuned_auto_panels();
+// Block pitch: 1 (default) fills windows up to the longest row, 0 half windows.
+int tuned_fill();
+// Forward window for plain CSR: 0 by value type, else 2048 (the plan's own
+// records) or 4096 (the wide records, where the plan holds them).
+int tuned_fwd_cap();
 
 // Stream-ordered scratch from the library's per-device pool.
 void *scratch_alloc(size_t bytes, hipStream_t s);
@@ -110,9 +119,13 @@ size_t mrhs_xr_bytes(int64_t ncols, int64_t k, int kp, int xtype);
 void launch_x_rowmajor(int xtype, const void *x, int64_t ldx, int64_t ncols,
                        int64_t k, int kp, void *xr, hipStream_t s);
 
+// fill: measure the longest row (waits for the stream) so that blocks fill
+// their windows (block_pitch); otherwise blocks start every half window.
 Plan *make_plan(int itype, const void *rowptr, int64_t base, int64_t nrows,
                 int64_t ncols, int64_t nnz, int sent, int64_t block_nnz,
-                hipStream_t s, bool validate, bool async_alloc);
+                hipStream_t s, bool validate, bool async_alloc, bool fill = false);
+// Entries between the starts of consecutive blocks of windows of cap entries.
+int64_t block_pitch(const Plan &p, int64_t cap);
 void free_plan(Plan *p, hipStream_t s);
 // Records a share of nnz entries needs with block_nnz-entry blocks (plus the
 // 2048-entry set of 1024-entry plans), sentinels included.
@@ -133,10 +146,13 @@ void run_spmv_panels(const Plan &p, int vtype, const void *colind,
                      const void *vals, int xtype, const void *x, int64_t ldx,
                      int ytype, void *y, int64_t ldy, int64_t k, double alpha,
                      double beta, hipStream_t s);
+// One column panel: its segments add alpha * sum into y[row]; a row's first
+// segment (row-map top bit) writes beta * y + alpha * sum instead.
 void run_spmv_rowmap(const Plan &p, int vtype, const void *colind,
                      const void *vals, const void *rowmap, int xtype,
                      const void *x, int64_t ldx, int ytype, void *y,
-                     int64_t ldy, int64_t k, double alpha, hipStream_t s);
+                     int64_t ldy, int64_t k, double alpha, double beta,
+                     hipStream_t s);
 void free_panels(PanelSet *ps);
 // A plan with panels is called with the colind/vals it was built from (or
 // last updated to); throws otherwise.

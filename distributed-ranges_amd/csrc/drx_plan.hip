@@ -65,6 +65,45 @@ __global__ void k_plan_records(const TI *rowptr, int64_t base, int64_t nrows,
   rec[c] = out;
 }
 
+// Longest row of the share (clamped to it): one partial maximum per
+// workgroup, then one atomic.
+template <typename TI>
+__global__ void k_max_rowlen(const TI *rowptr, int64_t base, int64_t nrows,
+                             int64_t nnz, int sent, unsigned long long *out) {
+  __shared__ unsigned long long red[4];
+  unsigned long long m = 0;
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < nrows;
+       r += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t len = row_begin(rowptr, base, nrows, nnz, sent, r + 1) -
+                        row_begin(rowptr, base, nrows, nnz, sent, r);
+    m = len > static_cast<int64_t>(m) ? static_cast<unsigned long long>(len) : m;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long v = __shfl_down(m, o, 64);
+    m = v > m ? v : m;
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < static_cast<int>(blockDim.x >> 6); i++) m = red[i] > m ? red[i] : m;
+    atomicMax(out, m);
+  }
+}
+
+// Entries between the starts of consecutive blocks of a window of `cap`
+// entries. Block c holds the rows whose first entry lies in
+// [c * pitch, (c + 1) * pitch), so it spans at most pitch + lmax - 1
+// entries: with the longest row known and at most cap / 2, the pitch fills
+// the window up to that row (cap - lmax); otherwise (unknown, long rows, or
+// fewer than 2.5 entries per row, whose blocks would hold more rows than the
+// LDS row table) it is cap / 2 and a last row that does not fit is reduced
+// by the whole workgroup. Knob "fill" 0 keeps cap / 2.
+int64_t block_pitch(const Plan &p, int64_t cap) {
+  if (tuned_fill() == 0 || p.lmax < 0 || p.lmax > cap / 2 || 2 * p.nnz < 5 * p.nrows)
+    return cap / 2;
+  return cap - std::max<int64_t>(p.lmax, 1);
+}
+
 int64_t default_block_nnz(int itype) {
   int64_t v = tuned_block_nnz();
   if (v > 0) return v;
@@ -72,10 +111,11 @@ int64_t default_block_nnz(int itype) {
   return 1024;
 }
 
-// Records of blocks of `nb` entries (window 2 nb) into a new array, or into
-// `into` (caller-owned, at least nblocks + 1 records) when it is given.
-static BlockRec *make_records(const Plan &p, int64_t nb, int64_t *nblocks, hipStream_t s,
-                              bool async_alloc, BlockRec *into = nullptr) {
+// Records of blocks starting every `nb` entries with windows of `cap`
+// entries into a new array, or into `into` (caller-owned, at least
+// nblocks + 1 records) when it is given.
+static BlockRec *make_records(const Plan &p, int64_t nb, int64_t cap, int64_t *nblocks,
+                              hipStream_t s, bool async_alloc, BlockRec *into = nullptr) {
   *nblocks = p.nnz / nb + 1;
   const size_t bytes = static_cast<size_t>(*nblocks + 1) * sizeof(BlockRec);
   BlockRec *rec = into;
@@ -90,11 +130,11 @@ static BlockRec *make_records(const Plan &p, int64_t nb, int64_t *nblocks, hipSt
   if (p.itype == DRX_I32)
     hipLaunchKernelGGL(k_plan_records<int32_t>, grid, dim3(tpb), 0, s,
                        static_cast<const int32_t *>(p.rowptr), p.base, p.nrows,
-                       p.nnz, p.has_sentinel, nb, 2 * nb, *nblocks, rec);
+                       p.nnz, p.has_sentinel, nb, cap, *nblocks, rec);
   else
     hipLaunchKernelGGL(k_plan_records<int64_t>, grid, dim3(tpb), 0, s,
                        static_cast<const int64_t *>(p.rowptr), p.base, p.nrows,
-                       p.nnz, p.has_sentinel, nb, 2 * nb, *nblocks, rec);
+                       p.nnz, p.has_sentinel, nb, cap, *nblocks, rec);
   DRX_HIP(hipGetLastError());
   return rec;
 }
@@ -102,12 +142,43 @@ static BlockRec *make_records(const Plan &p, int64_t nb, int64_t *nblocks, hipSt
 // The plan's records, and the records of twice as large blocks (window
 // 4096) that the forward kernel uses for 4-byte values when the plan's
 // block is 1024 entries.
+// The plan's records (windows of 2 block_nnz entries), and for the default
+// 1024-entry blocks the records of windows of 4096 that the forward kernel
+// uses for 4-byte values.
 static void build_records(Plan &p, hipStream_t s, bool async_alloc) {
   p.cap = 2 * p.block_nnz;
   p.owns_async = async_alloc;
-  p.rec = make_records(p, p.block_nnz, &p.nblocks, s, async_alloc);
+  p.rec = make_records(p, block_pitch(p, p.cap), p.cap, &p.nblocks, s, async_alloc);
   if (p.block_nnz == 1024)
-    p.rec_wide = make_records(p, 2048, &p.nblocks_wide, s, async_alloc);
+    p.rec_wide = make_records(p, block_pitch(p, 4096), 4096, &p.nblocks_wide, s, async_alloc);
+}
+
+// Longest row of p's share (waits for the stream).
+static int64_t max_row_len(const Plan &p, hipStream_t s) {
+  if (p.nrows == 0) return 0;
+  unsigned long long *d = static_cast<unsigned long long *>(
+      scratch_alloc(sizeof(unsigned long long), s));
+  unsigned long long h = 0;
+  try {
+    DRX_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), s));
+    const int64_t nb = std::min<int64_t>((p.nrows + 255) / 256, 4096);
+    if (p.itype == DRX_I32)
+      hipLaunchKernelGGL(k_max_rowlen<int32_t>, dim3(static_cast<unsigned>(nb)), dim3(256), 0, s,
+                         static_cast<const int32_t *>(p.rowptr), p.base, p.nrows, p.nnz,
+                         p.has_sentinel, d);
+    else
+      hipLaunchKernelGGL(k_max_rowlen<int64_t>, dim3(static_cast<unsigned>(nb)), dim3(256), 0, s,
+                         static_cast<const int64_t *>(p.rowptr), p.base, p.nrows, p.nnz,
+                         p.has_sentinel, d);
+    DRX_HIP(hipGetLastError());
+    DRX_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, s));
+    DRX_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    scratch_free(d, s);
+    throw;
+  }
+  scratch_free(d, s);
+  return static_cast<int64_t>(h);
 }
 
 int64_t records_needed(int64_t nnz, int64_t block_nnz) {
@@ -134,7 +205,7 @@ static void validate_plan_args(int itype, const void *rowptr, int64_t nrows,
 
 Plan *make_plan(int itype, const void *rowptr, int64_t base, int64_t nrows,
                 int64_t ncols, int64_t nnz, int sent, int64_t block_nnz,
-                hipStream_t s, bool validate, bool async_alloc) {
+                hipStream_t s, bool validate, bool async_alloc, bool fill) {
   validate_plan_args(itype, rowptr, nrows, ncols, nnz, block_nnz);
   Plan *p = new Plan();
   p->itype = itype;
@@ -174,9 +245,10 @@ Plan *make_plan(int itype, const void *rowptr, int64_t base, int64_t nrows,
         DRX_REQUIRE(lastv - base <= nnz,
                     "rowptr[nrows] - rowptr_base exceeds nnz");
     }
+    if (fill) p->lmax = max_row_len(*p, s);
     build_records(*p, s, async_alloc);
   } catch (...) {
-    delete p;
+    free_plan(p, s);
     throw;
   }
   return p;
@@ -199,8 +271,8 @@ void init_plan_in(Plan &p, int itype, const void *rowptr, int64_t base, int64_t 
   DRX_REQUIRE(records_needed(nnz, p.block_nnz) <= buf_records,
               "the share needs more plan records than the buffer holds");
   DRX_HIP(hipGetDevice(&p.device));
-  p.rec = make_records(p, p.block_nnz, &p.nblocks, s, false, buf);
-  p.rec_wide = make_records(p, 2048, &p.nblocks_wide, s, false, buf + p.nblocks + 1);
+  p.rec = make_records(p, p.block_nnz, p.cap, &p.nblocks, s, false, buf);
+  p.rec_wide = make_records(p, 2048, 4096, &p.nblocks_wide, s, false, buf + p.nblocks + 1);
 }
 
 void free_plan(Plan *p, hipStream_t s) {
@@ -235,7 +307,7 @@ int drx_plan_create(drx_plan *out, int itype, const void *rowptr,
     *out = nullptr;
     Plan *p = make_plan(itype, rowptr, rowptr_base, nrows, ncols, nnz,
                         has_sentinel, block_nnz,
-                        static_cast<hipStream_t>(stream), true, false);
+                        static_cast<hipStream_t>(stream), true, false, true);
     *out = reinterpret_cast<drx_plan>(p);
   });
 }

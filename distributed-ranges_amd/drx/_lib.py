@@ -45,6 +45,7 @@ SIGNATURES = {
     "drx_plan_info": (c_int, [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)]),
     "drx_plan_set_panels": (c_int, [c_vp, c_int, c_vp, c_vp, c_i64, c_vp]),
     "drx_plan_update_values": (c_int, [c_vp, c_vp, c_vp]),
+    "drx_plan_auto_panels": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, ctypes.POINTER(c_int)]),
     "drx_plan_panels": (c_int, [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64),
                                 ctypes.POINTER(c_i64)]),
     "drx_spmv": (c_int, [c_vp, c_int, c_vp, c_vp, c_int, c_vp, c_i64, c_int,
@@ -81,6 +82,7 @@ SIGNATURES = {
                              c_vp, c_u64, c_vp]),
     "drx_convert": (c_int, [c_int, c_vp, c_int, c_vp, c_i64, c_vp]),
     "drx_copy_device": (c_int, [c_vp, c_vp, c_i64, c_vp]),
+    "drx_stream_read": (c_int, [c_vp, c_i64, c_vp, c_i64, c_vp]),
 }
 
 

@@ -90,3 +90,14 @@ def gather(idx, x, out=None, stream=None):
     check(lib().drx_gather(_p(idx), idx.numel(), DTYPE_CODE[x.dtype], _p(x), x.numel(),
                            _p(out), _s(stream)))
     return out
+
+
+def stream_read(buf, digest=None, stream=None):
+    """Read every byte of the contiguous device tensor `buf` (drx_stream_read;
+    the benchmark's HBM read-rate probe). Returns the digest tensor."""
+    nbytes = buf.numel() * buf.element_size()
+    if digest is None:
+        digest = torch.empty(1 << 20, dtype=torch.int32, device=buf.device)
+    check(lib().drx_stream_read(_p(buf), nbytes - nbytes % 16, _p(digest), digest.numel(),
+                                _s(stream)))
+    return digest

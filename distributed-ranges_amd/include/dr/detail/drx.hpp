@@ -89,6 +89,35 @@ private:
   std::size_t n_ = 0;
 };
 
+// Pinned host memory (hipHostMalloc), grown on demand and kept: the staging
+// buffer of a rank's partial result, copied from the device at full link
+// speed and read in place by the root's assembly.
+template <typename T> class pinned_buffer {
+public:
+  pinned_buffer() = default;
+  pinned_buffer(const pinned_buffer &) = delete;
+  pinned_buffer &operator=(const pinned_buffer &) = delete;
+  ~pinned_buffer() { reset(); }
+  T *get(std::size_t n) {
+    if (n > n_) {
+      reset();
+      hip_check(hipHostMalloc(reinterpret_cast<void **>(&p_), n * sizeof(T), hipHostMallocDefault),
+                "hipHostMalloc");
+      n_ = n;
+    }
+    return p_;
+  }
+  void reset() {
+    if (p_) (void)hipHostFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+
+private:
+  T *p_ = nullptr;
+  std::size_t n_ = 0;
+};
+
 // A plan over one CSR share (drx_plan_create / drx_plan_destroy).
 class plan {
 public:
@@ -115,6 +144,18 @@ public:
     if (h_) drx_plan_destroy(h_);
   }
   drx_plan get() const noexcept { return h_; }
+  // Column-panel layout when it pays (drx_plan_auto_panels); true if built.
+  bool auto_panels(int vtype, const void *colind, const void *vals, hipStream_t s) {
+    int built = 0;
+    if (h_) check(drx_plan_auto_panels(h_, vtype, colind, vals, s, &built), "drx_plan_auto_panels");
+    return built != 0;
+  }
+  // Column panels of the plan (0 without the layout).
+  std::int64_t panels() const {
+    std::int64_t n = 0;
+    if (h_) check(drx_plan_panels(h_, &n, nullptr, nullptr), "drx_plan_panels");
+    return n;
+  }
 
 private:
   drx_plan h_ = nullptr;

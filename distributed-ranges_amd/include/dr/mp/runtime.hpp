@@ -106,6 +106,20 @@ public:
     barrier();
   }
 
+  // Gather without a copy: every rank publishes `src`; on root the result
+  // holds the ranks' pointers in rank order (elsewhere it is empty). The
+  // buffers must stay valid and unchanged until every rank has passed a
+  // later barrier (the gemv collection fences after the root has read them).
+  std::vector<const void *> gather_pointers(const void *src, std::size_t root) const {
+    auto &t = team();
+    t.slot[rank()] = src;
+    barrier();
+    std::vector<const void *> out;
+    if (rank() == root) out.assign(t.slot.begin(), t.slot.begin() + size());
+    barrier();
+    return out;
+  }
+
   // Every rank's `count` bytes, concatenated in rank order on every rank.
   void all_gather(const void *src, void *dst, std::size_t count) const {
     auto &t = team();

@@ -132,6 +132,10 @@ public:
   dr::index<std::size_t> shape() const { return shape_; }
   void fence() const { default_comm().barrier(); }
   std::size_t segment_size() const { return segment_size_; }
+  // Column panels of this rank's share plan (0: plain CSR). Not in the
+  // reference: the share's plan takes the panel layout when its columns
+  // scatter over an x larger than one XCD's L2 (drx_plan_auto_panels).
+  std::int64_t panels() const { return plan_.panels(); }
   const std::vector<std::size_t> &val_offsets() const { return val_offsets_; }
   const std::vector<std::size_t> &val_sizes() const { return val_sizes_; }
 
@@ -146,16 +150,19 @@ public:
                    "drx_spmv");
   }
 
+  // csr_row_distribution.hpp:136-153. The partial lives in pinned host
+  // memory kept across calls (the reference allocates and zero-fills it per
+  // call); ranks without rows contribute nothing the root reads.
   template <typename C>
   void local_gemv_and_collect(std::size_t root, C &res, T *x, std::size_t k) const {
     auto comm = default_comm();
     if (comm.rank() == root) assert(std::ranges::size(res) == shape_[0] * k);
     const std::size_t n_part = segment_size_ * k;
-    std::vector<T> partial(n_part, T(0));
+    T *partial = hpart_.get(std::max<std::size_t>(n_part, 1));
     if (rows_ > 0) {
       T *d = ws_.get(std::max<std::size_t>(n_part, 1));
       local_gemv(d, x, k);
-      dr::drx::hip_check(hipMemcpyAsync(partial.data(), d, n_part * sizeof(T),
+      dr::drx::hip_check(hipMemcpyAsync(partial, d, n_part * sizeof(T),
                                         hipMemcpyDeviceToHost, stream()), "D2H");
       dr::drx::hip_check(hipStreamSynchronize(stream()), "sync");
     }
@@ -168,20 +175,20 @@ public:
 private:
   // csr_row_distribution.hpp:158-204: Gather of equal-size partials, then the
   // root copies min(seg, m - j*seg) rows of every rank and right-hand side.
+  // The root reads each rank's partial in place (the shared address space
+  // is the transport), so there is no intermediate gather buffer.
   template <typename C>
-  void gather_gemv_vector(std::size_t root, C &res, const std::vector<T> &partial,
-                          std::size_t k) const {
+  void gather_gemv_vector(std::size_t root, C &res, const T *partial, std::size_t k) const {
     auto comm = default_comm();
     const std::size_t P = comm.size(), m = shape_[0];
-    std::vector<T> scratch(comm.rank() == root ? segment_size_ * P * k : 0);
-    comm.gather(partial.data(), scratch.data(), segment_size_ * k, root);
+    const auto parts = comm.gather_pointers(partial, root);
     if (comm.rank() != root) return;
     auto out = std::ranges::begin(res);
     for (std::size_t j = 0; j < P; j++) {
       if (j * segment_size_ >= m) break;
       const std::size_t css = std::min(segment_size_, m - j * segment_size_);
       for (std::size_t i = 0; i < k; i++) {
-        const T *piece = scratch.data() + j * k * segment_size_ + i * segment_size_;
+        const T *piece = static_cast<const T *>(parts[j]) + i * segment_size_;
         std::copy(piece, piece + css, out + m * i + j * segment_size_);
       }
     }
@@ -238,11 +245,13 @@ private:
     __detail::upload_share(vals + std::min(off, nnz_), colind + std::min(off, nnz_),
                            rowptr + std::min(segment_size_ * rank, m), my_nnz, rows_,
                            vals_, cols_, rowptr_);
-    if (rows_ > 0)
+    if (rows_ > 0) {
       plan_ = dr::drx::plan(rowptr_.data(), static_cast<std::int64_t>(off),
                             static_cast<std::int64_t>(rows_),
                             static_cast<std::int64_t>(shape_[1]),
                             static_cast<std::int64_t>(my_nnz), false, stream());
+      plan_.auto_panels(dr::drx::dtype_code<T>(), cols_.data(), vals_.data(), stream());
+    }
     __detail::share_desc<T, I> me{vals_.data(), cols_.data(), rowptr_.data(),
                                   static_cast<std::int64_t>(segment_size_ * rank),
                                   static_cast<std::int64_t>(rows_),
@@ -262,6 +271,7 @@ private:
   dr::drx::plan plan_;
   std::vector<__detail::share_desc<T, I>> shares_;
   mutable __detail::workspace<T> ws_;
+  mutable dr::drx::pinned_buffer<T> hpart_;  // the partial on the host
 };
 
 // ---------------------------------------------------------------------------
@@ -291,6 +301,10 @@ public:
   dr::index<std::size_t> shape() const { return shape_; }
   void fence() const { default_comm().barrier(); }
   std::size_t segment_size() const { return segment_size_; }
+  // Column panels of this rank's share plan (0: plain CSR). Not in the
+  // reference: the share's plan takes the panel layout when its columns
+  // scatter over an x larger than one XCD's L2 (drx_plan_auto_panels).
+  std::int64_t panels() const { return plan_.panels(); }
   const std::vector<std::size_t> &row_offsets() const { return row_offsets_; }
   const std::vector<std::size_t> &row_sizes() const { return row_sizes_; }
 
@@ -311,16 +325,18 @@ public:
                    "drx_spmv");
   }
 
+  // csr_eq_distribution.hpp:133-153, with the partial in pinned host memory
+  // kept across calls.
   template <typename C>
   void local_gemv_and_collect(std::size_t root, C &res, T *x, std::size_t k) const {
     auto comm = default_comm();
     if (comm.rank() == root) assert(std::ranges::size(res) == shape_[0] * k);
     const std::size_t n_part = row_sizes_[comm.rank()] * k;
-    std::vector<T> partial(n_part, T(0));
+    T *partial = hpart_.get(std::max<std::size_t>(n_part, 1));
     if (n_part) {
       T *d = ws_.get(n_part);
       local_gemv(d, x, k);
-      dr::drx::hip_check(hipMemcpyAsync(partial.data(), d, n_part * sizeof(T),
+      dr::drx::hip_check(hipMemcpyAsync(partial, d, n_part * sizeof(T),
                                         hipMemcpyDeviceToHost, stream()), "D2H");
       dr::drx::hip_check(hipStreamSynchronize(stream()), "sync");
     }
@@ -333,34 +349,33 @@ public:
 private:
   // csr_eq_distribution.hpp:158-216: Gatherv of [k][row_sizes[i]] partials;
   // the root zeroes res, adds every rank's first row (shared with the
-  // previous rank) and copies its other rows.
+  // previous rank) and copies its other rows. Here the root reads each
+  // rank's partial in place, and zeroes only what no copy writes first: the
+  // rows before a rank's first row that no earlier rank covered, that first
+  // row itself when the previous rank did not write it, and the rows after
+  // the last rank's. Every element ends with the bits the reference's
+  // zero-add-copy order gives (0 + a == a; boundary sums in rank order).
   template <typename C>
-  void gather_gemv_vector(std::size_t root, C &res, const std::vector<T> &partial,
-                          std::size_t k) const {
+  void gather_gemv_vector(std::size_t root, C &res, const T *partial, std::size_t k) const {
     auto comm = default_comm();
     const std::size_t P = comm.size(), m = shape_[0];
-    std::vector<long long> counts(P);
-    std::vector<long> offsets(P, 0);
-    for (std::size_t i = 0; i < P; i++)
-      counts[i] = static_cast<long long>(row_sizes_[i] * sizeof(T) * k);
-    for (std::size_t i = 0; i + 1 < P; i++) offsets[i + 1] = offsets[i] + counts[i];
-    std::vector<T> gathered(comm.rank() == root ? max_row_size_ * k : 0);
-    comm.gatherv(partial.data(), counts.data(), offsets.data(), gathered.data(), root);
+    const auto parts = comm.gather_pointers(partial, root);
     if (comm.rank() != root) return;
     auto out = std::ranges::begin(res);
-    std::fill(out, out + m * k, T(0));
     for (std::size_t kk = 0; kk < k; kk++) {
-      std::size_t cur = 0;
+      auto col = out + kk * m;
+      std::size_t written = 0;  // rows [0, written) of this column hold their value so far
       for (std::size_t i = 0; i < P; i++) {
-        const std::size_t first = row_offsets_[i], last = first + row_sizes_[i];
-        const std::size_t rs = row_sizes_[i];
-        if (first < last) out[first + kk * m] += gathered[k * cur + kk * rs];
-        if (first + 1 < last) {
-          const T *piece = gathered.data() + k * cur + kk * rs + 1;
-          std::copy(piece, piece + (last - first - 1), out + first + kk * m + 1);
-        }
-        cur += row_sizes_[i];
+        const std::size_t first = std::min(row_offsets_[i], m);
+        const std::size_t last = std::min(first + row_sizes_[i], m);
+        if (first >= last) continue;
+        const T *piece = static_cast<const T *>(parts[i]) + kk * row_sizes_[i];
+        if (written <= first) std::fill(col + written, col + first + 1, T(0));
+        col[first] += piece[0];
+        std::copy(piece + 1, piece + (last - first), col + first + 1);
+        written = std::max(written, last);
       }
+      if (written < m) std::fill(col + written, col + m, T(0));
     }
   }
 
@@ -422,11 +437,13 @@ private:
     const std::size_t rows_avail = std::min(rows, m - rfirst);
     __detail::upload_share(vals + lo, colind + lo, rowptr + rfirst, my_nnz, rows_avail,
                            vals_, cols_, rowptr_);
-    if (my_nnz > 0)
+    if (my_nnz > 0) {
       plan_ = dr::drx::plan(rowptr_.data(), static_cast<std::int64_t>(lo),
                             static_cast<std::int64_t>(rows),
                             static_cast<std::int64_t>(shape_[1]),
                             static_cast<std::int64_t>(my_nnz), false, stream());
+      plan_.auto_panels(dr::drx::dtype_code<T>(), cols_.data(), vals_.data(), stream());
+    }
     __detail::share_desc<T, I> me{vals_.data(), cols_.data(), rowptr_.data(),
                                   static_cast<std::int64_t>(rfirst),
                                   static_cast<std::int64_t>(rows_avail),
@@ -446,6 +463,7 @@ private:
   dr::drx::plan plan_;
   std::vector<__detail::share_desc<T, I>> shares_;
   mutable __detail::workspace<T> ws_;
+  mutable dr::drx::pinned_buffer<T> hpart_;  // the partial on the host
 };
 
 // ---------------------------------------------------------------------------

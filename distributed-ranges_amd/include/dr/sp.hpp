@@ -249,6 +249,28 @@ OutIt copy(It first, It last, OutIt out) {
   return copy<T, OutIt>(first, last, out);
 }
 
+// Host -> device copy of the contiguous host range [first, last) into a
+// distributed vector from position `out` on.
+template <std::contiguous_iterator InIt, typename It>
+  requires requires(It i) { i.vec(); i.pos(); }
+It copy(InIt first, InIt last, It out) {
+  using T = typename std::iterator_traits<It>::value_type;
+  auto segs = out.vec()->segments();
+  const std::size_t seg = out.vec()->segment_size();
+  const std::size_t total = static_cast<std::size_t>(last - first);
+  const T *src = std::to_address(first);
+  std::size_t done = 0;
+  for (std::size_t i = out.pos(); done < total;) {
+    const std::size_t s = i / seg, off = i % seg;
+    const std::size_t n = std::min(segs[s].size - off, total - done);
+    dr::drx::hip_check(hipMemcpy(segs[s].data + off, src + done, n * sizeof(T),
+                                 hipMemcpyHostToDevice), "H2D");
+    done += n;
+    i += n;
+  }
+  return out + static_cast<std::ptrdiff_t>(total);
+}
+
 template <typename T> class duplicated_vector {
 public:
   using value_type = T;
@@ -315,7 +337,25 @@ public:
     tl.vals.upload(v.values_data(), tl.nnz);
     tl.colind.upload(v.colind_data(), tl.nnz);
     tl.rowptr.upload(v.rowptr_data(), tl.tm + 1);
+    tl.plan = dr::drx::plan();
     snapshot_.reset();
+  }
+
+  // The kept plan of tile t (built on first use on stream q, then reused by
+  // every product with the tile until copy_tile replaces it). The
+  // reference's tiles persist across calls likewise
+  // (sparse_matrix.hpp:193-207); only oneMKL's handle is rebuilt per call
+  // (local_gemv.hpp:63-72), which this plan replaces.
+  drx_plan tile_plan(key_type t, hipStream_t q) const {
+    const std::size_t idx = t[0] * grid_[1] + t[1];
+    const auto &tl = tiles_[idx];
+    if (tl.plan.get() == nullptr && tl.tm > 0) {
+      tl.plan = dr::drx::plan(tl.rowptr.data(), 0, static_cast<std::int64_t>(tl.tm),
+                              static_cast<std::int64_t>(tl.tn),
+                              static_cast<std::int64_t>(tl.nnz), true, q);
+      tl.plan.auto_panels(dr::drx::dtype_code<T>(), tl.colind.data(), tl.vals.data(), q);
+    }
+    return tl.plan.get();
   }
 
   // Entries in tile order with global indices (downloads the tiles once).
@@ -327,6 +367,7 @@ private:
     std::size_t tm = 0, tn = 0, nnz = 0, rank = 0, row0 = 0, col0 = 0;
     dr::drx::device_buffer<T> vals;
     dr::drx::device_buffer<I> rowptr, colind;
+    mutable dr::drx::plan plan;  // built by tile_plan on first use
   };
 
   static std::unique_ptr<matrix_partition> default_partition() {
@@ -469,6 +510,20 @@ void local_gemv(hipStream_t q, dr::views::csr_matrix_view<T, I> a, const T *b, T
                               static_cast<std::int64_t>(a.shape()[0]), 1, 1.0, 1.0, q),
                  "drx_spmv_csr");
 }
+
+// c[0 .. tm) += A_tile b through the tile's kept plan.
+template <typename T, typename I>
+void local_gemv(hipStream_t q, dr::views::csr_matrix_view<T, I> a, drx_plan plan,
+                const T *b, T *c) {
+  if (a.shape()[0] == 0) return;
+  dr::drx::check(drx_spmv(plan, dr::drx::dtype_code<T>(), a.colind_data(),
+                          a.values_data(),
+                          dr::drx::dtype_code<T>(), b,
+                          static_cast<std::int64_t>(a.shape()[1]),
+                          dr::drx::dtype_code<T>(), c,
+                          static_cast<std::int64_t>(a.shape()[0]), 1, 1.0, 1.0, q),
+                 "drx_spmv");
+}
 }  // namespace __detail
 
 // c += A b with row tiles ({P, 1} grid): b is replicated onto every device
@@ -495,8 +550,9 @@ void gemv(distributed_vector<T> &c, sparse_matrix<T, I> &a, distributed_vector<T
   for (std::size_t i = 0; i < a.grid_shape()[0]; i++) {
     auto t = a.tile(dr::index<I>(i, 0));
     assert(csegs[i].size == t.shape()[0]);
-    __detail::local_gemv(__detail::queue(t.rank()), t, scratch.local_vector(t.rank()).data(),
-                         csegs[i].data);
+    hipStream_t q = __detail::queue(t.rank());
+    __detail::local_gemv(q, t, a.tile_plan(dr::index<I>(i, 0), q),
+                         scratch.local_vector(t.rank()).data(), csegs[i].data);
   }
   __detail::wait_all();
 }
@@ -539,7 +595,7 @@ void gemv_tiles_2d(distributed_vector<T> &c, sparse_matrix<T, I> &a, BPtr &&b_of
       if (prev) dr::drx::hip_check(hipStreamWaitEvent(q, prev, 0), "hipStreamWaitEvent");
       const T *bk = b_of(i, k, q, copy_segments);
       assert(csegs[i].size == t.shape()[0]);
-      local_gemv(q, t, bk, csegs[i].data);
+      local_gemv(q, t, a.tile_plan(dr::index<I>(i, k), q), bk, csegs[i].data);
       hipEvent_t e = nullptr;
       dr::drx::hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
       dr::drx::hip_check(hipEventRecord(e, q), "hipEventRecord");

@@ -99,6 +99,17 @@ const char *drx_last_error(void);
  *                 rest from memory); 0 and 1 (default) gather x per entry,
  *                 which measured faster on bands. Results are the same bit
  *                 for bit.
+ *   "auto_panels" 1 (default): drx_plan_auto_panels builds the panel layout
+ *                 when it pays; 0: never.
+ *   "fill"        1 (default): plans that measure their longest row L
+ *                 (drx_plan_create, transposes, panels, batches) start a
+ *                 block every W - L entries for a window of W entries, so
+ *                 blocks fill their windows (when L <= W / 2 and rows
+ *                 average at least 2.5 entries); 0: every W / 2 entries.
+ *   "fwd_cap"     forward window of plain CSR with one right-hand side: 0
+ *                 (default) 4096 for FP32 values with 32-bit indices (see
+ *                 "wide"), else the plan's own (2 block_nnz); 2048 or 4096
+ *                 force one where the plan holds those records.
  *   "gemm_tile"   drx_gemm workgroup tile: 1 128 x 128; 2 256 x 128; 3 and
  *                 4 the same with stages twice as deep in k; 5 256 x 256
  *                 (FP32; FP64 takes 4); 6 128 x 128 loading stages straight
@@ -111,12 +122,14 @@ int drx_tune_set(const char *key, int64_t value);
 int drx_tune_get(const char *key, int64_t *value);
 
 /* ---- plan: analysis of one CSR share ------------------------------------
- * Splits the share's rows into blocks of about `block_nnz` entries (rows
- * whose first entry falls in one block_nnz-sized window of the share; a row
- * longer than the window is reduced by a whole workgroup). block_nnz <= 0
+ * Splits the share's rows into blocks for windows of 2 * block_nnz entries
+ * (block c: the rows whose first entry falls in [c * pitch, (c + 1) *
+ * pitch); the pitch is the window minus the share's longest row when that
+ * row fits half a window, else half a window, and a last row that would
+ * overflow the window is reduced by the whole workgroup). block_nnz <= 0
  * picks the tuned default. Keeps a reference to `rowptr` (the caller keeps it
  * alive and unchanged). Synchronizes `stream` once to validate the share's
- * bounds.
+ * bounds and measure its longest row.
  * Replaces the per-share setup of csr_row_distribution::init
  * (csr_row_distribution.hpp:205-318), csr_eq_distribution::init
  * (csr_eq_distribution.hpp:220-333) and oneMKL's set_csr_data for the sp
@@ -146,6 +159,21 @@ int drx_plan_update_values(drx_plan plan, const void *vals,
                            drx_stream_t stream);
 int drx_plan_panels(drx_plan plan, int64_t *npanels, int64_t *panel_cols,
                     int64_t *segments);
+/* Builds the panel layout (as drx_plan_set_panels with panel_cols = 0) when
+ * it pays, and reports it in *built (1, else 0; may be NULL). It pays when
+ * x of vtype exceeds 4 MiB (one XCD's L2), the share averages at least 4
+ * entries per row, its columns scatter (fewer than half of the entries
+ * sampled from up to 2048 row blocks lie within 1 MiB of x of their block's
+ * middle column; one small kernel), the share holds at most 2^31 - 1
+ * entries and the device has memory for the copy. Synchronizes the stream;
+ * refuses a stream under capture. Knob "auto_panels" 0 turns it off. Called
+ * by the host interface's share and tile setup (the seams of
+ * csr_row_distribution::init, csr_row_distribution.hpp:205-318,
+ * csr_eq_distribution::init, csr_eq_distribution.hpp:220-333, and the
+ * sp tiles, sparse_matrix.hpp:193-207), so that their products reach the
+ * panel kernels. */
+int drx_plan_auto_panels(drx_plan plan, int vtype, const void *colind,
+                         const void *vals, drx_stream_t stream, int *built);
 
 /* ---- forward ------------------------------------------------------------
  * y[:, j] = alpha * A x[:, j] + beta * y[:, j]   for j in [0, k)
@@ -337,6 +365,13 @@ int drx_copy_device(void *dst, const void *src, int64_t bytes,
  * random-gather rate that bounds products with scattered columns. */
 int drx_gather(const int32_t *idx, int64_t n, int xtype, const void *x,
                int64_t nx, void *out, drx_stream_t stream);
+/* Reads `bytes` (a multiple of 16, src 16-byte aligned) from device memory
+ * with streaming 16-byte loads and writes one 4-byte digest per workgroup
+ * into digest[0 .. min(digest_len, workgroups)). No counterpart in the
+ * reference: the benchmark times it over a buffer much larger than the
+ * caches to measure the HBM read ceiling that bounds the products. */
+int drx_stream_read(const void *src, int64_t bytes, void *digest,
+                    int64_t digest_len, drx_stream_t stream);
 /* Convert between storage dtypes (round to nearest even). */
 int drx_convert(int src_dtype, const void *src, int dst_dtype, void *dst,
                 int64_t n, drx_stream_t stream);

@@ -3,6 +3,8 @@
 // P = 1, 2, 3, 4 ranks sharing the GPU, plus the reference benchmark's
 // GemvEq_DR input (band matrix, width 8, x[i + j*n] = i*j + 1) checked for
 // exact equality against a host loop (examples/mp/sparse_matrix.cpp:84-102).
+#include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <iterator>
 #include <ranges>
@@ -199,6 +201,104 @@ TEST(SparseMatrix, Int32IndicesEmptyRows) {
   std::allocator<double>{}.deallocate(csr.values_data(), csr.size());
   std::allocator<int>{}.deallocate(csr.rowptr_data(), m + 1);
   std::allocator<int>{}.deallocate(csr.colind_data(), csr.size());
+}
+
+// A Zipf(1.5) matrix (rows clipped to [1, lmax], stratified sorted columns,
+// values N(0, 1)) generated on the GPU by the library's generators and
+// copied into host arrays (release with dr::destroy_csr_matrix_view).
+static dr::views::csr_matrix_view<float, int> zipf_host(int m, int n, int lmax,
+                                                        std::uint64_t seed) {
+  dr::drx::device_buffer<int> rp(m + 1);
+  std::int64_t nnz = 0;
+  dr::drx::check(drx_gen_zipf_rowptr(DRX_I32, rp.data(), m, 1.5, lmax, seed, &nnz, nullptr),
+                 "drx_gen_zipf_rowptr");
+  dr::drx::device_buffer<int> ci(nnz);
+  dr::drx::device_buffer<float> v(nnz);
+  dr::drx::check(drx_gen_stratified_colind(DRX_I32, rp.data(), m, n, ci.data(), seed + 1, nullptr),
+                 "drx_gen_stratified_colind");
+  dr::drx::check(drx_gen_normal(DRX_F32, v.data(), nnz, seed + 2, nullptr), "drx_gen_normal");
+  int *hrp = new int[m + 1];
+  int *hci = new int[nnz];
+  float *hv = new float[nnz];
+  rp.download(hrp, m + 1);
+  ci.download(hci, nnz);
+  v.download(hv, nnz);
+  return dr::views::csr_matrix_view<float, int>(hv, hrp, hci, dr::index<int>(m, n), nnz, 0);
+}
+
+// Componentwise error max |y - y*| / (|A||x|) against a long double loop.
+static double zipf_err(const dr::views::csr_matrix_view<float, int> &a,
+                       const std::vector<float> &x, const std::vector<float> &y) {
+  const int m = static_cast<int>(a.shape()[0]);
+  double worst = 0;
+  for (int r = 0; r < m; r++) {
+    long double s = 0, t = 0;
+    for (int q = a.rowptr_data()[r]; q < a.rowptr_data()[r + 1]; q++) {
+      const long double p = static_cast<long double>(a.values_data()[q]) * x[a.colind_data()[q]];
+      s += p;
+      t += p < 0 ? -p : p;
+    }
+    const double d = std::abs(static_cast<double>(y[r] - s));
+    if (t > 0) worst = std::max(worst, d / static_cast<double>(t));
+    else if (d != 0) worst = 1;
+  }
+  return worst;
+}
+
+// The headline speed through the reference's seams: a share whose columns
+// scatter over an x larger than one XCD's L2 (2M columns of FP32, 8 MB)
+// takes the column-panel layout when its distribution builds the share
+// (csr_row_distribution::init / csr_eq_distribution::init), and
+// dr::mp::gemv then runs the panel kernels; results within the FP32
+// tolerance (componentwise 1e-5) of a long double loop, for P = 1 and 2.
+TEST(SparseMatrix, ZipfShareTakesColumnPanels) {
+  const int m = 2000000, n = 2000000;
+  auto csr = zipf_host(m, n, 256, 77);
+  std::vector<float> x(n);
+  for (int j = 0; j < n; j++) x[j] = static_cast<float>(std::sin(0.001 * j + 0.3));
+  for (std::size_t P = 1; P <= 2; P++) {
+    mp::run_ranks(P, [&] {
+      mp::distributed_sparse_matrix<float, int, mp::MpiBackend,
+                                    mp::csr_row_distribution<float, int, mp::MpiBackend>>
+          r(csr, 0);
+      mp::distributed_sparse_matrix<float, int> e(csr, 0);  // csr_eq
+      EXPECT_TRUE(r.matrix_distribution().panels() >= 4);
+      EXPECT_TRUE(e.matrix_distribution().panels() >= 4);
+      mp::broadcasted_vector<float> b;
+      b.broadcast_data(n, 0, x, mp::default_comm());
+      std::vector<float> yr(m), ye(m);
+      mp::gemv(yr, r, b);
+      mp::gemv(0, ye, e, b);
+      EXPECT_TRUE(zipf_err(csr, x, yr) < 1e-5);
+      if (mp::default_comm().rank() == 0) EXPECT_TRUE(zipf_err(csr, x, ye) < 1e-5);
+      b.destroy_data();
+    });
+  }
+  // the knob turns the layout off; results stay within tolerance
+  dr::drx::check(drx_tune_set("auto_panels", 0), "drx_tune_set");
+  mp::run_ranks(1, [&] {
+    mp::distributed_sparse_matrix<float, int> e(csr, 0);
+    EXPECT_TRUE(e.matrix_distribution().panels() == 0);
+    mp::broadcasted_vector<float> b;
+    b.broadcast_data(n, 0, x, mp::default_comm());
+    std::vector<float> y(m);
+    mp::gemv(y, e, b);
+    EXPECT_TRUE(zipf_err(csr, x, y) < 1e-5);
+    b.destroy_data();
+  });
+  dr::drx::check(drx_tune_set("auto_panels", 1), "drx_tune_set");
+  dr::destroy_csr_matrix_view(csr);
+}
+
+// Local columns (a band) over the same x size keep plain CSR.
+TEST(SparseMatrix, BandShareKeepsPlainCsr) {
+  const long n = 2000000;
+  auto csr = dr::generate_band_csr<float, int>(n, 8, 8);
+  mp::run_ranks(1, [&] {
+    mp::distributed_sparse_matrix<float, int> e(csr, 0);
+    EXPECT_TRUE(e.matrix_distribution().panels() == 0);
+  });
+  dr::destroy_csr_matrix_view(csr);
 }
 
 int main(int argc, char **argv) { return harness::run_all(argc > 1 ? argv[1] : nullptr); }

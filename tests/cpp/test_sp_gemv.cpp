@@ -2,6 +2,8 @@
 // (test/gtest/sp/gemv.cpp:7-184) replayed through this host interface with
 // 1, 2, 3 and 4 logical devices on the one GPU (the reference's
 // --devicesCount duplication, test/gtest/sp/sp-tests.cpp:31-42).
+#include <cmath>
+#include <cstdint>
 #include <vector>
 
 #include "dr/sp.hpp"
@@ -131,6 +133,60 @@ TEST(SparseMatrix, GemvSquareCopy) {
 }
 TEST(SparseMatrix, FlatGemv) {
   square_case(300, 500, [](auto &c, auto &a, auto &b) { sp::flat_gemv(c, a, b); }, false);
+}
+
+// Tiles keep their plans across products, and a tile whose columns scatter
+// over an x larger than one XCD's L2 (2M columns of FP32) takes the column
+// panel layout: two products with the same b add up to twice one product
+// (c += A b accumulates), within the FP32 tolerance of a long double loop.
+TEST(SparseMatrix, ZipfTilesTakeColumnPanels) {
+  const int m = 1000000, n = 2000000;
+  dr::drx::device_buffer<int> rp(m + 1);
+  std::int64_t nnz = 0;
+  dr::drx::check(drx_gen_zipf_rowptr(DRX_I32, rp.data(), m, 1.5, 256, 5, &nnz, nullptr), "zipf");
+  dr::drx::device_buffer<int> ci(nnz);
+  dr::drx::device_buffer<float> v(nnz);
+  dr::drx::check(drx_gen_stratified_colind(DRX_I32, rp.data(), m, n, ci.data(), 6, nullptr), "cols");
+  dr::drx::check(drx_gen_normal(DRX_F32, v.data(), nnz, 7, nullptr), "vals");
+  std::vector<int> hrp(m + 1), hci(nnz);
+  std::vector<float> hv(nnz);
+  rp.download(hrp.data(), m + 1);
+  ci.download(hci.data(), nnz);
+  v.download(hv.data(), nnz);
+  dr::views::csr_matrix_view<float, int> csr(hv.data(), hrp.data(), hci.data(),
+                                             dr::index<int>(m, n), nnz, 0);
+  std::vector<float> x(n);
+  for (int j = 0; j < n; j++) x[j] = static_cast<float>(std::cos(0.002 * j));
+  for (std::size_t P = 1; P <= 2; P++) {
+    sp::init(P);
+    auto a = sp::create_distributed(csr, sp::row_cyclic());
+    sp::distributed_vector<float> b(n, 0.f);
+    sp::copy(x.begin(), x.end(), b.begin());
+    sp::distributed_vector<float> c(m, 0.f);
+    sp::gemv(c, a, b);
+    for (std::size_t i = 0; i < a.grid_shape()[0]; i++) {
+      std::int64_t np = 0;
+      dr::drx::check(drx_plan_panels(a.tile_plan(dr::index<int>(i, 0), nullptr), &np, nullptr,
+                                     nullptr), "drx_plan_panels");
+      EXPECT_TRUE(np >= 4);
+    }
+    sp::gemv(c, a, b);
+    std::vector<float> y(m);
+    sp::copy(c.begin(), c.end(), y.begin());
+    double worst = 0;
+    for (int r = 0; r < m; r++) {
+      long double s = 0, t = 0;
+      for (int q = hrp[r]; q < hrp[r + 1]; q++) {
+        const long double p = static_cast<long double>(hv[q]) * x[hci[q]];
+        s += p;
+        t += p < 0 ? -p : p;
+      }
+      const double d = std::abs(static_cast<double>(y[r] - 2 * s));
+      if (t > 0) worst = std::max(worst, d / static_cast<double>(2 * t));
+    }
+    EXPECT_TRUE(worst < 2e-5);
+    sp::finalize();
+  }
 }
 
 int main(int argc, char **argv) { return harness::run_all(argc > 1 ? argv[1] : nullptr); }
