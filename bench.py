@@ -504,6 +504,9 @@ def main():
     ap.add_argument("--xstage", type=int, default=None,
                     help="x windows in LDS (the library's xstage knob: 0 never, 1 default, "
                          "2 always); for profiling the forms with and without")
+    ap.add_argument("--vec", type=int, default=None,
+                    help="consecutive entries per lane and load (the library's vec knob: 0 "
+                         "default, 1, 2, 4); for profiling the forms with each")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 2 GiB device copy 5 times (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -525,6 +528,8 @@ def main():
     torch.cuda.set_device(0)
     if args.xstage is not None:
         drx.tune_set("xstage", args.xstage)
+    if args.vec is not None:
+        drx.tune_set("vec", args.vec)
     A, x, y, B, wall_ms, ev_ms, layout, gather = headline(args.steps, args.warmup)
     value = B / (wall_ms * 1e-3) / 1e9
     achieved = B / (ev_ms * 1e-3) / 1e9

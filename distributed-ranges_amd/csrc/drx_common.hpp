@@ -175,19 +175,19 @@ __device__ __forceinline__ void wave_min_max(uint32_t &lo, uint32_t &hi) {
   hi = h;
 }
 
-// Smallest and largest valid column index of a workgroup's window: lane
-// entries tid + u * WG < n (c[u] for u < T). xmm holds 2 * WG / 64 words of
-// LDS. Ends after a barrier with the same lo, hi in every lane (lo > hi when
-// the window is empty).
-template <int WG, int T>
-__device__ __forceinline__ void block_col_range(const uint32_t (&c)[T], int n, uint32_t *xmm,
+// Smallest and largest column index of a workgroup's window over the slots
+// u (c[u] for u < T) for which in_block(u) holds. xmm holds 2 * WG / 64
+// words of LDS. Ends after a barrier with the same lo, hi in every lane (lo >
+// hi when the window is empty).
+template <int WG, int T, typename F>
+__device__ __forceinline__ void block_col_range(const uint32_t (&c)[T], F in_block, uint32_t *xmm,
                                                 uint32_t &lo, uint32_t &hi) {
   const int tid = threadIdx.x;
   lo = 0xffffffffu;
   hi = 0;
 #pragma unroll
   for (int u = 0; u < T; u++) {
-    if (tid + u * WG < n) {
+    if (in_block(u)) {
       lo = c[u] < lo ? c[u] : lo;
       hi = c[u] > hi ? c[u] : hi;
     }
@@ -203,6 +203,14 @@ __device__ __forceinline__ void block_col_range(const uint32_t (&c)[T], int n, u
     lo = xmm[w] < lo ? xmm[w] : lo;
     hi = xmm[WG / 64 + w] > hi ? xmm[WG / 64 + w] : hi;
   }
+}
+
+// The same over lane entries tid + u * WG < n.
+template <int WG, int T>
+__device__ __forceinline__ void block_col_range(const uint32_t (&c)[T], int n, uint32_t *xmm,
+                                                uint32_t &lo, uint32_t &hi) {
+  block_col_range<WG, T>(c, [&](int u) { return static_cast<int>(threadIdx.x) + u * WG < n; },
+                         xmm, lo, hi);
 }
 
 // ---------------------------------------------------------------------------
@@ -224,6 +232,12 @@ __device__ __forceinline__ int64_t row_begin(const TI *rowptr, int64_t base,
 // Top bit of a column panel's row-map entry: the segment is its row's first
 // in panel order (row ids are non-negative, so the bit is free).
 template <typename TI> constexpr TI kFirstSeg = static_cast<TI>(TI(1) << (sizeof(TI) * 8 - 1));
+
+// Entries a block's window leaves free: a window of W entries holds at most
+// W - kWinSlack of the block's entries, so that lanes streaming groups of up
+// to kWinSlack consecutive entries from the group-aligned entry below the
+// block's first cover the whole block within W slots.
+constexpr int kWinSlack = 4;
 
 // Number of XCDs that a launch's workgroups are dealt over round-robin.
 constexpr int kXcds = 8;
@@ -250,13 +264,20 @@ __device__ __forceinline__ rsrc_t make_rsrc(const void *base, uint32_t bytes) {
 
 template <typename T, int AUX = 0>
 __device__ __forceinline__ T buf_load(rsrc_t r, uint32_t off) {
-  if constexpr (sizeof(T) == 8)
+  if constexpr (sizeof(T) == 16)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, AUX));
+  else if constexpr (sizeof(T) == 8)
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, AUX));
   else if constexpr (sizeof(T) == 4)
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, AUX));
   else
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, AUX));
 }
+
+// N consecutive 4-byte words (one buffer load of 4 N bytes).
+template <int N> struct Words {
+  uint32_t w[N];
+};
 
 // Column index as a 32-bit element offset (the kernels that use buffer
 // gathers require ncols * sizeof(x) < 2^32).

@@ -180,21 +180,25 @@ template <typename TV, typename U> __device__ __forceinline__ TV bits_to(U u) {
 // One row block `blk` of the share described by `a` (all lanes of the
 // workgroup call it together).
 //
-// VW = 2 (4-byte or smaller values, 32-bit indices, plain or row-map mode):
-// each lane streams pairs of consecutive entries with one 8-byte colind load
-// and one 4- or 8-byte vals load, from the pair-aligned entry below the
-// window start (the caller guarantees pair-aligned vals / colind pointers).
-// The window's last pair may extend one entry past the window (and past the
-// arrays' last entry) inside that same aligned pair; such entries are read
-// but never used.
+// VW = 2 or 4 (32-bit indices, plain or row-map mode, VW values in at most
+// 16 bytes): each lane streams groups of VW consecutive entries, one 4 VW-
+// byte colind load and one VW * sizeof(TV)-byte vals load per group, from the
+// group-aligned entry ea below the block's first entry e0 (the caller
+// guarantees vals / colind pointers aligned to a group). Slot s of the LDS
+// product buffer holds entry ea + s; the block's entries are slots
+// [e0 - ea, e0 - ea + n_s), inside the buffer because a window leaves
+// kWinSlack entries free. The last group may extend up to VW - 1 entries
+// past the block (and past the arrays' last entry, inside that same aligned
+// group); such entries are read but never used.
 template <int WG, int CAP, int MODE, bool BUFX, typename TV, typename TX,
           typename TY, typename TA, typename TI, int VW = 1, bool XS = false>
 __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
                                               FwdSmem<WG, CAP, MODE == kRowMap, TA, TI> &sm) {
-  static_assert(VW == 1 || (VW == 2 && MODE != kPerm && sizeof(TI) == 4 && sizeof(TV) <= 4),
-                "pairs of entries: plain or row-map mode, 32-bit indices, <= 4-byte values");
-  static_assert(!XS || (MODE == kPlain && BUFX && VW == 1),
-                "x staging: plain mode, buffer gathers, one entry per load");
+  static_assert(VW == 1 || ((VW == 2 || VW == 4) && MODE != kPerm && sizeof(TI) == 4 &&
+                            VW * sizeof(TV) <= 16 && VW <= kWinSlack),
+                "groups of entries: plain or row-map mode, 32-bit indices, at most 16 bytes "
+                "of values per group");
+  static_assert(!XS || (MODE == kPlain && BUFX), "x staging: plain mode, buffer gathers");
   constexpr int T = CAP / WG;
   constexpr int ROWCAP = CAP / 2;
   TA *prod = sm.prod;
@@ -218,13 +222,14 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
     const int n_s = static_cast<int>(b.es - b.e0);
     const int64_t nr = (b.es == e1 ? r1 : r1 - 1) - b.r0;
     const bool rows_lds = nr < ROWCAP;
-    // entry (relative to e0) held in slot u of a lane
+    // LDS slot of entry u of a lane (the entry is ea + slot, ea = e0 - sh);
+    // the block's entries are slots [sh, sh + n_s)
     const int sh = VW > 1 ? static_cast<int>(b.e0 & (VW - 1)) : 0;
-    auto eidx = [&](int u) -> int {
+    auto slot = [&](int u) -> int {
       if constexpr (VW == 1)
         return tid + u * WG;
       else
-        return (tid + (u / VW) * WG) * VW + (u % VW) - sh;
+        return (tid + (u / VW) * WG) * VW + (u % VW);
     };
 
     constexpr bool kRowsFirst = MODE == kRowMap && sizeof(TV) == 4;
@@ -287,34 +292,25 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
     // Stream the window (reads past n_s return 0 and touch no memory).
     TA v[T];
     uint32_t c[T];
-    // VW = 2: the lanes' pairs cover entries [-sh, CAP - sh); a full window
-    // (n_s == CAP) starting at an odd entry leaves entry CAP - 1 to lane 0.
-    const bool tail = VW == 2 && n_s + sh > CAP;
-    TA v_tail = 0;
-    uint32_t c_tail = 0;
-    if constexpr (VW == 2) {
-      if (tail && tid == 0) {
-        v_tail = to_acc<TA>(vals[b.e0 + CAP - 1]);
-        c_tail = col32(colind[b.e0 + CAP - 1]);
-      }
+    if constexpr (VW > 1) {
+      // the lanes' groups cover entries [ea, ea + CAP) and the block's
+      // entries [e0, e0 + n_s) with sh + n_s <= CAP
       const int64_t ea = b.e0 - sh;
-      const uint32_t npair = static_cast<uint32_t>(n_s + sh + 1) / 2;
-      const rsrc_t rv = make_rsrc(vals + ea, npair * 2 * sizeof(TV));
-      const rsrc_t rc = make_rsrc(colind + ea, npair * 2 * sizeof(TI));
+      const uint32_t ngrp = static_cast<uint32_t>(n_s + sh + VW - 1) / VW;
+      constexpr uint32_t VB = VW * sizeof(TV);  // bytes of a group's values
+      const rsrc_t rv = make_rsrc(vals + ea, ngrp * VB);
+      const rsrc_t rc = make_rsrc(colind + ea, ngrp * VW * 4);
 #pragma unroll
-      for (int p = 0; p < T / 2; p++) {
-        const uint32_t pi = tid + p * WG;
-        const uint64_t cw = buf_load<uint64_t, kNT>(rc, pi * 8);
-        c[2 * p] = static_cast<uint32_t>(cw);
-        c[2 * p + 1] = static_cast<uint32_t>(cw >> 32);
-        if constexpr (sizeof(TV) == 4) {
-          const uint64_t w = buf_load<uint64_t, kNT>(rv, pi * 8);
-          v[2 * p] = to_acc<TA>(bits_to<TV>(static_cast<uint32_t>(w)));
-          v[2 * p + 1] = to_acc<TA>(bits_to<TV>(static_cast<uint32_t>(w >> 32)));
-        } else {
-          const uint32_t w = buf_load<uint32_t, kNT>(rv, pi * 4);
-          v[2 * p] = to_acc<TA>(bits_to<TV>(w & 0xffffu));
-          v[2 * p + 1] = to_acc<TA>(bits_to<TV>(w >> 16));
+      for (int g = 0; g < T / VW; g++) {
+        const uint32_t gi = tid + g * WG;
+        const Words<VW> cw = buf_load<Words<VW>, kNT>(rc, gi * (VW * 4));
+        const Words<(VB + 3) / 4> vw = buf_load<Words<(VB + 3) / 4>, kNT>(rv, gi * VB);
+        TV t[VW];
+        __builtin_memcpy(t, vw.w, VB);
+#pragma unroll
+        for (int i = 0; i < VW; i++) {
+          c[g * VW + i] = cw.w[i];
+          v[g * VW + i] = to_acc<TA>(t[i]);
         }
       }
     } else {
@@ -374,8 +370,12 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
     if constexpr (XS) {
       if (a.k == 1) {
         constexpr uint32_t XCAP = CAP * sizeof(TA) / sizeof(TX);
+        auto in_block = [&](int u) {
+          const int sl = slot(u);
+          return sl >= sh && sl < sh + n_s;
+        };
         uint32_t lo, hi;
-        block_col_range<WG, T>(c, n_s, sm.xmm, lo, hi);
+        block_col_range<WG, T>(c, in_block, sm.xmm, lo, hi);
         if (lo <= hi) {
           const uint32_t xn = hi - lo + 1 < XCAP ? hi - lo + 1 : XCAP;
           TX *xs = reinterpret_cast<TX *>(prod);
@@ -383,24 +383,25 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
           for (uint32_t i = tid; i < xn; i += WG) xs[i] = buf_load<TX>(rx, (lo + i) * sizeof(TX));
           __syncthreads();
           // All LDS reads first, then the rare columns past the window from
-          // memory, then the products (slots past n_s hold 0 * x[lo]).
+          // memory, then the products (slots outside the block hold
+          // products the reduction never reads).
           TA xv[T];
           bool far = false;
 #pragma unroll
           for (int u = 0; u < T; u++) {
             const uint32_t off = c[u] - lo;
             xv[u] = to_acc<TA>(xs[off < xn ? off : 0]);
-            far |= off >= xn && tid + u * WG < n_s;
+            far |= off >= xn && in_block(u);
           }
           if (far) {
 #pragma unroll
             for (int u = 0; u < T; u++)
-              if (c[u] - lo >= xn && tid + u * WG < n_s)
+              if (c[u] - lo >= xn && in_block(u))
                 xv[u] = to_acc<TA>(buf_load<TX>(rx, c[u] * sizeof(TX)));
           }
           __syncthreads();
 #pragma unroll
-          for (int u = 0; u < T; u++) prod[tid + u * WG] = v[u] * xv[u];
+          for (int u = 0; u < T; u++) prod[slot(u)] = v[u] * xv[u];
           staged = true;
         }
       }
@@ -420,35 +421,33 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
           // the path below.
 #pragma unroll
           for (int u = 0; u < T; u++) {
-            const int idx = eidx(u);
-            if (idx >= 0 && idx < n_s)
-              prod[idx] = v[u] * to_acc<TA>(buf_load<TX>(rx, c[u] * sizeof(TX)));
+            const int s = slot(u);
+            if (s >= sh && s < sh + n_s)
+              prod[s] = v[u] * to_acc<TA>(buf_load<TX>(rx, c[u] * sizeof(TX)));
           }
         } else {
           // Every lane issues all T gathers before storing a product: a
           // per-entry predicate around load and store made the compiler
           // wait for each gather in turn (band +-16 FP32: 338 -> 273 us).
-          // Slots past the window gather x[0] (their column reads 0) into
-          // product slots that the reduction never reads; the pair slot
-          // before a window (idx = -1, lane 0) goes to CAP - 1, which only
-          // the tail entry -- stored after it by the same lane -- uses.
+          // Slots outside the block (past the window: their column reads 0,
+          // so they gather x[0]; before e0 in a group: an entry of the
+          // previous block) go to product slots that the reduction never
+          // reads.
           TA xg[T];
 #pragma unroll
           for (int u = 0; u < T; u++) xg[u] = to_acc<TA>(buf_load<TX>(rx, c[u] * sizeof(TX)));
 #pragma unroll
-          for (int u = 0; u < T; u++) prod[(eidx(u) + CAP) & (CAP - 1)] = v[u] * xg[u];
+          for (int u = 0; u < T; u++) prod[slot(u)] = v[u] * xg[u];
         }
-        if (tail && tid == 0) prod[CAP - 1] = v_tail * to_acc<TA>(buf_load<TX>(rx, c_tail * sizeof(TX)));
       } else {
 #pragma unroll
         for (int u = 0; u < T; u++) {
-          const int idx = eidx(u);
-          if (idx >= 0 && idx < n_s) prod[idx] = v[u] * to_acc<TA>(xj[c[u]]);
+          const int s = slot(u);
+          if (s >= sh && s < sh + n_s) prod[s] = v[u] * to_acc<TA>(xj[c[u]]);
         }
-        if (tail && tid == 0) prod[CAP - 1] = v_tail * to_acc<TA>(xj[c_tail]);
       }
       __syncthreads();
-      reduce_rows_any<WG, MODE, TA, TY, TI>(n_s, prod, rp_s, map_s,
+      reduce_rows_any<WG, MODE, TA, TY, TI>(n_s, prod + sh, rp_s, map_s,
                                             MODE == kRowMap && a.k == 1 ? ypre : nullptr,
                                             rows_lds, a, b.r0, b.e0,
                                       nr, y + j * a.ldy, alpha, beta);

@@ -386,7 +386,8 @@ int drx_tune_set(const char *key, int64_t value) {
       DRX_REQUIRE(value == 0 || value == 1, "wide must be 0 or 1");
       g_wide.store(static_cast<int>(value));
     } else if (std::strcmp(key, "vec") == 0) {
-      DRX_REQUIRE(value >= 0 && value <= 2, "vec must be 0, 1 or 2");
+      DRX_REQUIRE(value == 0 || value == 1 || value == 2 || value == 4,
+                  "vec must be 0, 1, 2 or 4");
       g_vec.store(static_cast<int>(value));
     } else if (std::strcmp(key, "grid_cap") == 0) {
       DRX_REQUIRE(value >= 0 && value <= (int64_t(1) << 30), "grid_cap out of range");

@@ -302,20 +302,20 @@ static void build_panels(Plan &p, PanelSet &ps, const TI *colind,
     cnt = nullptr;
     // One plan per panel over its segments (absolute entry offsets, sentinel
     // = next panel's first entry). The plan's base is the panel's first
-    // entry rounded down to an even entry, so that the panel's arrays start
-    // pair-aligned (two entries per lane per load); a panel starting at an
-    // odd entry then has one leading entry that belongs to none of its rows.
+    // entry rounded down to a multiple of 4, so that the panel's arrays start
+    // aligned to a group of entries (2 or 4 entries per lane per load); a
+    // panel starting elsewhere then has up to 3 leading entries that belong
+    // to none of its rows.
     const TI *segp = static_cast<const TI *>(ps.seg_ptr);
-    // A panel without entries gets no plan (the product skips it): with an
-    // odd start its rounded-down base would give it one stray entry and no
-    // rows.
+    // A panel without entries gets no plan (the product skips it): its
+    // rounded-down base would give it stray entries and no rows.
     for (int64_t q = 0; q < ps.npanels; q++) {
       const int64_t rows = ps.pseg[q + 1] - ps.pseg[q];
       if (ps.pstart[q + 1] == ps.pstart[q]) {
         ps.plans.push_back(nullptr);
         continue;
       }
-      const int64_t base = ps.pstart[q] & ~int64_t(1);
+      const int64_t base = ps.pstart[q] & ~int64_t(3);
       const int64_t pn = ps.pstart[q + 1] - base;
       ps.plans.push_back(make_plan(p.itype, segp + ps.pseg[q], base, rows, p.ncols, pn, 1,
                                    p.block_nnz, s, false, false, true));
@@ -401,7 +401,7 @@ void run_spmv_panels(const Plan &p, int vtype, const void *colind,
   const size_t isz = p.itype == DRX_I32 ? 4 : 8, vsz = dtype_size(vtype);
   for (int64_t q = 0; q < ps.npanels; q++) {
     if (ps.pstart[q + 1] == ps.pstart[q]) continue;
-    const int64_t base = ps.pstart[q] & ~int64_t(1);  // the panel plan's base
+    const int64_t base = ps.pstart[q] & ~int64_t(3);  // the panel plan's base
     run_spmv_rowmap(*ps.plans[q], vtype,
                     static_cast<const char *>(ps.pcol) + base * isz,
                     static_cast<const char *>(ps.pval) + base * vsz,

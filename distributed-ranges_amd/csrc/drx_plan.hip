@@ -1,13 +1,13 @@
 // drx_plan.hip -- the plan of one CSR share (row blocks and their entry
 // windows), library status strings, and the plan entry points.
 //
-// Block c holds the rows whose first entry lies in [c*NB, (c+1)*NB) of the
-// share (NB = block_nnz), found by one binary search on rowptr per block
-// boundary. Its record also stores where the LDS window ends (es): all of
-// the block's entries when they fit 2*NB, otherwise everything before the
-// block's last row, which is then longer than NB and is reduced by the whole
-// workgroup. Computing this once lets the kernels start streaming vals and
-// colind as soon as they read their 32-byte record.
+// Block c holds the rows whose first entry lies in [c*P, (c+1)*P) of the
+// share (P = the pitch, block_pitch), found by one binary search on rowptr
+// per block boundary. Its record also stores where the LDS window ends (es):
+// all of the block's entries when they fit the window less its slack
+// (kWinSlack), otherwise everything before the block's last row, which is
+// then reduced by the whole workgroup. Computing this once lets the kernels
+// start streaming vals and colind as soon as they read their 32-byte record.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -57,7 +57,7 @@ __global__ void k_plan_records(const TI *rowptr, int64_t base, int64_t nrows,
   if (c < nblocks) {
     const int64_t r1 = r_of(c + 1);
     const int64_t e1 = row_begin(rowptr, base, nrows, nnz, sent, r1);
-    if (r1 > r0 && e1 - e0 > cap)
+    if (r1 > r0 && e1 - e0 > cap - kWinSlack)
       out.es = row_begin(rowptr, base, nrows, nnz, sent, r1 - 1);
     else
       out.es = e1;
@@ -94,14 +94,15 @@ __global__ void k_max_rowlen(const TI *rowptr, int64_t base, int64_t nrows,
 // entries. Block c holds the rows whose first entry lies in
 // [c * pitch, (c + 1) * pitch), so it spans at most pitch + lmax - 1
 // entries: with the longest row known and at most cap / 2, the pitch fills
-// the window up to that row (cap - lmax); otherwise (unknown, long rows, or
-// fewer than 2.5 entries per row, whose blocks would hold more rows than the
-// LDS row table) it is cap / 2 and a last row that does not fit is reduced
-// by the whole workgroup. Knob "fill" 0 keeps cap / 2.
+// the window up to that row and the window's slack (cap - kWinSlack -
+// lmax); otherwise (unknown, long rows, or fewer than 2.5 entries per row,
+// whose blocks would hold more rows than the LDS row table) it is cap / 2
+// and a last row that does not fit is reduced by the whole workgroup. Knob
+// "fill" 0 keeps cap / 2.
 int64_t block_pitch(const Plan &p, int64_t cap) {
   if (tuned_fill() == 0 || p.lmax < 0 || p.lmax > cap / 2 || 2 * p.nnz < 5 * p.nrows)
     return cap / 2;
-  return cap - std::max<int64_t>(p.lmax, 1);
+  return cap - kWinSlack - std::max<int64_t>(p.lmax, 1);
 }
 
 int64_t default_block_nnz(int itype) {

@@ -68,12 +68,16 @@ static void launch_fwd_vw(int64_t cap, const FwdArgs &a, hipStream_t s) {
                        grid, dim3(1024), 0, s, a);
 }
 
-// Pairs of entries per lane (VW = 2) for 4-byte-or-smaller values and
-// 32-bit indices when vals / colind are aligned to a pair. Knob "vec": 0
-// (default) pairs in column panels (where it measured faster), 1 never
-// pairs, 2 pairs wherever eligible. With every gather of a lane in flight,
-// single entries beat pairs for 16-bit values too (3-D Laplacian 256^3:
-// BF16 323 vs 358 us, FP16 351 vs 358 us).
+// Groups of VW consecutive entries per lane (32-bit indices, VW values in at
+// most 16 bytes, vals / colind aligned to a group; not the permuted-values
+// mode). Knob "vec": 0 (default) by type: groups of four for values of at
+// most 4 bytes, single entries for FP64; 1 single entries; 2 pairs; 4
+// groups of four, wherever eligible (else single entries). Measured with
+// every gather in flight (us per call): 3-D Laplacian 256^3 FP32 271 -> 222,
+// BF16 / FP16 234 / 260 -> 199, headline Zipf column panels 8.44 -> 8.12 ms
+// per step; band +-16 FP32 211 -> 212 (its gathers, coalesced across lanes
+// with one entry per lane, spread over four times the lines with groups);
+// FP64 pairs (3-D Laplacian 403 -> 420) spill.
 template <int MODE, bool BUFX, typename TV, typename TX, typename TY,
           typename TA, typename TI>
 static void launch_fwd(const Plan &p, FwdArgs a, hipStream_t s) {
@@ -103,21 +107,44 @@ static void launch_fwd(const Plan &p, FwdArgs a, hipStream_t s) {
       cap = want;
     }
   }
-  if constexpr (MODE != kPerm && sizeof(TI) == 4 && sizeof(TV) <= 4) {
-    const bool aligned =
-        reinterpret_cast<uintptr_t>(a.vals) % (2 * sizeof(TV)) == 0 &&
-        reinterpret_cast<uintptr_t>(a.colind) % 8 == 0;
+  int vw = 1;
+  if constexpr (MODE != kPerm && sizeof(TI) == 4) {
     const int vec = tuned_vec();
-    const bool want = vec == 2 || (vec == 0 && MODE == kRowMap);
-    if (aligned && want) {
+    const int want = vec == 0 ? (sizeof(TV) <= 4 ? 4 : 1) : vec;
+    auto aligned = [&](int w) {
+      return reinterpret_cast<uintptr_t>(a.vals) % (w * sizeof(TV)) == 0 &&
+             reinterpret_cast<uintptr_t>(a.colind) % (w * 4) == 0;
+    };
+    if (want == 4 && 4 * sizeof(TV) <= 16 && aligned(4))
+      vw = 4;
+    else if (want == 2 && 2 * sizeof(TV) <= 16 && aligned(2))
+      vw = 2;
+  }
+  // x staged in LDS (knob xstage 2): plain CSR with x of the values' type
+  // (instantiated apart, so the other kernels keep their registers).
+  constexpr bool kCanStage = MODE == kPlain && BUFX && std::is_same_v<TV, TX>;
+  if constexpr (kCanStage && sizeof(TI) == 4 && 4 * sizeof(TV) <= 16) {
+    if (vw == 4) {
+      if (a.xstage)
+        launch_fwd_vw<MODE, BUFX, TV, TX, TY, TA, TI, 4, true>(cap, a, s);
+      else
+        launch_fwd_vw<MODE, BUFX, TV, TX, TY, TA, TI, 4>(cap, a, s);
+      return;
+    }
+  }
+  if constexpr (MODE != kPerm && sizeof(TI) == 4 && 4 * sizeof(TV) <= 16) {
+    if (vw == 4) {
+      launch_fwd_vw<MODE, BUFX, TV, TX, TY, TA, TI, 4>(cap, a, s);
+      return;
+    }
+  }
+  if constexpr (MODE != kPerm && sizeof(TI) == 4 && 2 * sizeof(TV) <= 16) {
+    if (vw == 2) {
       launch_fwd_vw<MODE, BUFX, TV, TX, TY, TA, TI, 2>(cap, a, s);
       return;
     }
   }
-  // x staged in LDS: FP64 and FP32 plain CSR only (instantiated apart, so
-  // the other kernels keep their registers).
-  if constexpr (MODE == kPlain && BUFX && std::is_same_v<TV, TX> &&
-                (std::is_same_v<TV, float> || std::is_same_v<TV, double>)) {
+  if constexpr (kCanStage) {
     if (a.xstage) {
       launch_fwd_vw<MODE, BUFX, TV, TX, TY, TA, TI, 1, true>(cap, a, s);
       return;

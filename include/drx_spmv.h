@@ -82,11 +82,11 @@ const char *drx_last_error(void);
  *                 8 for k > 8 over loosely local columns -- stencils,
  *                 bands -- else 16; batches use 16). Fixed per call, and per
  *                 batch when it is created.
- *   "vec"         entries per lane of the streaming loads for <= 4-byte
- *                 values with 32-bit indices: 0 (default) two in column
- *                 panels, one otherwise; 1 always one; 2 two wherever
- *                 eligible (pairs need vals and colind aligned to a pair of
- *                 entries; others use one).
+ *   "vec"         consecutive entries per lane and load (32-bit indices,
+ *                 one right-hand side or column gathers): 0 (default) two
+ *                 in column panels, one otherwise; 1 always one; 2 pairs;
+ *                 4 groups of four (values of at most 4 bytes) -- wherever
+ *                 eligible (vals and colind aligned to a group), else one.
  *   "wide"        1 (default): FP32 values with 32-bit indices stream
  *                 2048-entry blocks (16 per lane) of plans whose block_nnz
  *                 is the default 1024; 0: the plan's own blocks.

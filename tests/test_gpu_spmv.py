@@ -563,43 +563,53 @@ def _odd_window_rows(nb):
     return [nb + 1, nb - 2, nb + 2, 0, 3, 1, 300, 7, 0, 129, 5, 64, 63, 1, 2, 9000, 11]
 
 
-@pytest.mark.parametrize("vdt", [torch.float32, torch.bfloat16, torch.float16])
+@pytest.mark.parametrize("vdt", [torch.float32, torch.bfloat16, torch.float16, torch.float64])
 @pytest.mark.parametrize("block_nnz", [64, 512, 1024])
 def test_paired_entry_loads_bitexact(orc, vdt, block_nnz):
-    """vec = 2 (two entries per lane per load) gives the same bits as one
-    entry per lane: full windows starting at odd entries, unaligned vals
-    (one-element offset: falls back to single entries), column panels."""
+    """vec = 2 and 4 (groups of two or four consecutive entries per lane and
+    load, from the group-aligned entry below each block) give the same bits
+    as one entry per lane: windows starting at every offset within a group,
+    full windows, vals starting one to three elements into a larger buffer
+    (not group-aligned: single entries), column panels. FP64 values take
+    pairs (four would exceed 16 bytes per load: single entries)."""
     rng = np.random.default_rng(block_nnz)
     lens = _odd_window_rows(block_nnz) + list(rng.integers(0, 40, 3000))
     n = 20000
     rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
     ci = np.concatenate([np.sort(rng.choice(n, L, replace=False)) for L in lens]).astype(np.int64)
     v = rng.uniform(-1, 1, ci.size)
-    Ad = upload(rp, ci, v.astype(np.float32), (len(lens), n), block_nnz=block_nnz)
+    Ad = upload(rp, ci, v.astype(np.float64 if vdt == torch.float64 else np.float32),
+                (len(lens), n), block_nnz=block_nnz)
     vq = Ad.vals.to(vdt)
-    x = drx.gen.uniform(n, torch.float32, -1, 1, seed=3)
-    xq = x.to(vdt) if vdt != torch.float32 else x
+    xdt = torch.float64 if vdt == torch.float64 else torch.float32
+    x = drx.gen.uniform(n, xdt, -1, 1, seed=3)
+    xq = x.to(vdt) if vdt not in (torch.float32, torch.float64) else x
     with drx.tuned(vec=1):
         y1 = Ad.matvec(xq, vals=vq)
-    with drx.tuned(vec=2):
-        y2 = Ad.matvec(xq, vals=vq)
-        # vals starting one element into a larger buffer: not pair-aligned
-        buf = torch.zeros(vq.numel() + 1, dtype=vdt, device=DEV)
-        buf[1:] = vq
-        y3 = Ad.matvec(xq, vals=buf[1:])
+    ys_all = []
+    for vec in (2, 4):
+        with drx.tuned(vec=vec):
+            ys_all.append(Ad.matvec(xq, vals=vq))
+            for off in (1, 2, 3):  # vals not aligned to a group: single entries
+                buf = torch.zeros(vq.numel() + off, dtype=vdt, device=DEV)
+                buf[off:] = vq
+                ys_all.append(Ad.matvec(xq, vals=buf[off:]))
     torch.cuda.synchronize()
-    assert torch.equal(y1, y2) and torch.equal(y1, y3)
+    for yv in ys_all:
+        assert torch.equal(y1, yv)
     ys, absy = orc.spmv(rp, ci, vq.double().cpu().numpy(), xq.double().cpu().numpy())
-    assert componentwise_err(y2.cpu().numpy(), ys, absy) < 1e-5
+    tol = 1e-12 if vdt == torch.float64 else 1e-5
+    assert componentwise_err(ys_all[0].cpu().numpy(), ys, absy) < tol
     if vdt == torch.float32:
         Ad.set_panels(4096)
-        with drx.tuned(vec=1):
-            p1 = Ad.matvec(x)
-        with drx.tuned(vec=2):
-            p2 = Ad.matvec(x)
+        pv = []
+        for vec in (1, 2, 4):
+            with drx.tuned(vec=vec):
+                pv.append(Ad.matvec(x))
         torch.cuda.synchronize()
-        assert torch.equal(p1, p2)
-        assert componentwise_err(p2.cpu().numpy(), ys, absy) < 1e-5
+        assert torch.equal(pv[0], pv[1]) and torch.equal(pv[0], pv[2])
+        assert componentwise_err(pv[2].cpu().numpy(), ys, absy) < 1e-5
+    Ad.close()
 
 
 def _staging_cases(rng):
