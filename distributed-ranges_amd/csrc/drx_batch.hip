@@ -71,6 +71,7 @@ struct Batch {
   int num_cus = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  int vw = 1;                   // entries per lane and load of forward items
   bool mrhs = false;            // k >= 2 through row-major chunks of x
   int kp = 0;                   // their width, fixed when the batch is made
   void *xr_all = nullptr;       // all items' chunks
@@ -98,8 +99,10 @@ template <int WG, typename TA> constexpr int batch_waves() {
   return (sizeof(TA) == 8 || WG >= 512) ? 4 : 5;
 }
 
+// VW: consecutive entries per lane and load of the forward items (4 when
+// every forward item's vals / colind are aligned to a group of four).
 template <int WG, int CAP, typename TV, typename TX, typename TY, typename TA,
-          typename TI>
+          typename TI, int VW>
 __global__ __launch_bounds__(WG)
 __attribute__((amdgpu_waves_per_eu(batch_waves<WG, TA>(), 8)))
 void k_batch(const FwdArgs *items,
@@ -137,14 +140,18 @@ void k_batch(const FwdArgs *items,
     if (mode == kPerm)
       process_block<WG, CAP, kPerm, true, TV, TX, TY, TA, TI>(cur, blk, sm);
     else
-      process_block<WG, CAP, kPlain, true, TV, TX, TY, TA, TI>(cur, blk, sm);
+      process_block<WG, CAP, kPlain, true, TV, TX, TY, TA, TI, VW>(cur, blk, sm);
   }
 }
 
 template <int WG, int CAP, typename TV, typename TX, typename TY, typename TA,
           typename TI>
-static auto batch_kernel() {
-  return k_batch<WG, CAP, TV, TX, TY, TA, TI>;
+static const void *batch_kernel(int vw) {
+  if constexpr (sizeof(TI) == 4 && 4 * sizeof(TV) <= 16) {
+    if (vw == 4) return reinterpret_cast<const void *>(k_batch<WG, CAP, TV, TX, TY, TA, TI, 4>);
+  }
+  (void)vw;
+  return reinterpret_cast<const void *>(k_batch<WG, CAP, TV, TX, TY, TA, TI, 1>);
 }
 
 // k >= 2: the same walk over (item, block) with the multi-RHS block body.
@@ -259,13 +266,13 @@ static void pick_kernel_t(const Batch &b, const void **fn, int *wg) {
     else
       pick_mrhs_kp<16, TV, TX, TY, TA, TI>(b, fn, wg);
   } else if (cap <= 1024) {
-    *fn = reinterpret_cast<const void *>(batch_kernel<256, 1024, TV, TX, TY, TA, TI>());
+    *fn = batch_kernel<256, 1024, TV, TX, TY, TA, TI>(b.vw);
     *wg = 256;
   } else if (cap <= 2048) {
-    *fn = reinterpret_cast<const void *>(batch_kernel<256, 2048, TV, TX, TY, TA, TI>());
+    *fn = batch_kernel<256, 2048, TV, TX, TY, TA, TI>(b.vw);
     *wg = 256;
   } else {
-    *fn = reinterpret_cast<const void *>(batch_kernel<512, 4096, TV, TX, TY, TA, TI>());
+    *fn = batch_kernel<512, 4096, TV, TX, TY, TA, TI>(b.vw);
     *wg = 512;
   }
 }
@@ -420,6 +427,19 @@ static void replan(Batch &b, int64_t block_nnz, hipStream_t s) {
     off[i + 1] = off[i] + (p->nrows > 0 ? p->nblocks : 0);
   }
   b.total = off[n];
+  // Groups of four consecutive entries per lane and load for the forward
+  // items (knob vec 0 or 4) when their type allows it and every forward
+  // item's vals / colind are aligned to a group.
+  {
+    const size_t sv = dtype_size(b.vtype);
+    const int vec = tuned_vec();
+    bool four = (vec == 0 || vec == 4) && b.itype == DRX_I32 && 4 * sv <= 16;
+    for (int64_t i = 0; four && i < n; i++)
+      if (modes[i] == kPlain)
+        four = reinterpret_cast<uintptr_t>(args[i].vals) % (4 * sv) == 0 &&
+               reinterpret_cast<uintptr_t>(args[i].colind) % 16 == 0;
+    b.vw = four ? 4 : 1;
+  }
   DRX_HIP(hipMalloc(&b.d_args, std::max<int64_t>(n, 1) * sizeof(FwdArgs)));
   DRX_HIP(hipMalloc(&b.d_mode, std::max<int64_t>(n, 1) * sizeof(int)));
   DRX_HIP(hipMalloc(&b.d_off, (n + 1) * sizeof(int64_t)));
