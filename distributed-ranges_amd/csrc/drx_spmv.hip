@@ -37,13 +37,14 @@ Grid grid_for(int64_t nblocks) {
   return g;
 }
 
-// Window 4096 on 256 lanes (16 entries per lane) for 4-byte values and
-// indices in plain mode (the permuted-values variant spills registers):
-// measured faster than 512 lanes x 8 (band +-16 FP32: 3544 vs 3006
-// GB/s) and than 1024-entry blocks (3-D Laplacian FP32: 3369 vs 3094).
+// Window 4096 on 256 lanes (16 entries per lane) for values of at most 4
+// bytes with 32-bit indices in plain mode (the permuted-values variant
+// spills registers): for FP32 measured faster than 512 lanes x 8 (band
+// +-16: 3544 vs 3006 GB/s) and than 1024-entry blocks (3-D Laplacian: 3369
+// vs 3094). 16-bit values take it when knob fwd_cap asks for 4096.
 template <int MODE, typename TV, typename TA, typename TI>
 constexpr bool wide_window() {
-  return MODE == kPlain && sizeof(TV) == 4 && sizeof(TA) == 4 && sizeof(TI) == 4;
+  return MODE == kPlain && sizeof(TV) <= 4 && sizeof(TA) == 4 && sizeof(TI) == 4;
 }
 
 template <int MODE, bool BUFX, typename TV, typename TX, typename TY,
@@ -82,6 +83,19 @@ template <int MODE, bool BUFX, typename TV, typename TX, typename TY,
           typename TA, typename TI>
 static void launch_fwd(const Plan &p, FwdArgs a, hipStream_t s) {
   int64_t cap = p.cap;
+  int vw = 1;
+  if constexpr (MODE != kPerm && sizeof(TI) == 4) {
+    const int vec = tuned_vec();
+    const int want = vec == 0 ? (sizeof(TV) <= 4 ? 4 : 1) : vec;
+    auto aligned = [&](int w) {
+      return reinterpret_cast<uintptr_t>(a.vals) % (w * sizeof(TV)) == 0 &&
+             reinterpret_cast<uintptr_t>(a.colind) % (w * 4) == 0;
+    };
+    if (want == 4 && 4 * sizeof(TV) <= 16 && aligned(4))
+      vw = 4;
+    else if (want == 2 && 2 * sizeof(TV) <= 16 && aligned(2))
+      vw = 2;
+  }
   if constexpr (MODE == kPlain) {
     // Window per value type (knob fwd_cap overrides): 4096 for 4-byte
     // values and indices (16 entries per lane; knob wide = 0 keeps the
@@ -90,8 +104,12 @@ static void launch_fwd(const Plan &p, FwdArgs a, hipStream_t s) {
     // FP64 2048: 400 / 388 us, 1024 (every gather in flight): 472 / 449 us,
     // 4096 (512 lanes): 436 / 405 us; BF16 2048: 234 us, 1024: 306 us,
     // 4096: 263 us; FP32 band 4096: 217 us, 2048: 244 us.
+    // 16-bit values take 4096 with groups of four (band +-16 BF16: 189 vs
+    // 195 us; 3-D Laplacian equal), 2048 with single entries (4096: 296 us).
     int64_t want = tuned_fwd_cap();
-    if (want == 0 && wide_window<MODE, TV, TA, TI>() && tuned_wide() != 0) want = 4096;
+    if (want == 0 && wide_window<MODE, TV, TA, TI>() && tuned_wide() != 0 &&
+        (sizeof(TV) == 4 || vw == 4))
+      want = 4096;
     const BlockRec *rec = nullptr;
     int64_t nb = 0;
     if (want == 4096 && p.rec_wide != nullptr) {
@@ -106,19 +124,6 @@ static void launch_fwd(const Plan &p, FwdArgs a, hipStream_t s) {
       a.grid = g.grid;
       cap = want;
     }
-  }
-  int vw = 1;
-  if constexpr (MODE != kPerm && sizeof(TI) == 4) {
-    const int vec = tuned_vec();
-    const int want = vec == 0 ? (sizeof(TV) <= 4 ? 4 : 1) : vec;
-    auto aligned = [&](int w) {
-      return reinterpret_cast<uintptr_t>(a.vals) % (w * sizeof(TV)) == 0 &&
-             reinterpret_cast<uintptr_t>(a.colind) % (w * 4) == 0;
-    };
-    if (want == 4 && 4 * sizeof(TV) <= 16 && aligned(4))
-      vw = 4;
-    else if (want == 2 && 2 * sizeof(TV) <= 16 && aligned(2))
-      vw = 2;
   }
   // x staged in LDS (knob xstage 2): plain CSR with x of the values' type
   // (instantiated apart, so the other kernels keep their registers).

@@ -83,13 +83,14 @@ const char *drx_last_error(void);
  *                 bands -- else 16; batches use 16). Fixed per call, and per
  *                 batch when it is created.
  *   "vec"         consecutive entries per lane and load (32-bit indices,
- *                 one right-hand side or column gathers): 0 (default) two
- *                 in column panels, one otherwise; 1 always one; 2 pairs;
- *                 4 groups of four (values of at most 4 bytes) -- wherever
- *                 eligible (vals and colind aligned to a group), else one.
- *   "wide"        1 (default): FP32 values with 32-bit indices stream
- *                 2048-entry blocks (16 per lane) of plans whose block_nnz
- *                 is the default 1024; 0: the plan's own blocks.
+ *                 one right-hand side or column gathers): 0 (default) groups
+ *                 of four for values of at most 4 bytes, one for FP64; 1
+ *                 one; 2 pairs; 4 groups of four (values of at most 4
+ *                 bytes) -- wherever eligible (vals and colind aligned to a
+ *                 group), else one. Same bits for every choice.
+ *   "wide"        1 (default): the windows of 4096 entries (16 per lane)
+ *                 that "fwd_cap" 0 picks, for plans whose block_nnz is the
+ *                 default 1024; 0: the plan's own blocks.
  *   "grid_cap"    most workgroups of a per-share launch (0 = one per row
  *                 block); capped launches loop over the blocks.
  *   "xstage"      x windows in LDS: 2 makes each row block of the forward
@@ -107,7 +108,8 @@ const char *drx_last_error(void);
  *                 blocks fill their windows (when L <= W / 2 and rows
  *                 average at least 2.5 entries); 0: every W / 2 entries.
  *   "fwd_cap"     forward window of plain CSR with one right-hand side: 0
- *                 (default) 4096 for FP32 values with 32-bit indices (see
+ *                 (default) 4096 for FP32 values, and for 16-bit values
+ *                 streamed in groups of four, with 32-bit indices (see
  *                 "wide"), else the plan's own (2 block_nnz); 2048 or 4096
  *                 force one where the plan holds those records.
  *   "gemm_tile"   drx_gemm workgroup tile: 1 128 x 128; 2 256 x 128; 3 and
