@@ -120,6 +120,29 @@ def headline(steps, warmup):
     return A, x, y, B, wall_ms, ev_ms, layout, gather
 
 
+def form_panel_sweep(A, x, steps, widths=(262144, 393216, 524288, 786432, 1048576)):
+    """The headline product through column panels of each width (columns of x
+    per panel): time per step and the row segments the layout makes. Narrow
+    panels cost row-segment work; a panel whose x slice and streamed entries
+    outgrow an XCD's 4 MB L2 misses on its gathers."""
+    n = A.shape[0]
+    y = torch.empty(n, dtype=torch.float32, device="cuda")
+    B = spmv_bytes(n, A.shape[1], A.nnz, 4, 4, 4, 4)
+    out = []
+    for w in widths:
+        M = drx.CsrMatrix(A.rowptr, A.colind, A.vals, A.shape)
+        npan, width, nseg = M.set_panels(w)
+        _, ms = time_launches(lambda: M.matvec(x, y=y), steps, 2)
+        out.append({"panel_cols": width, "MB_of_x": round(width * 4 / 2**20, 3),
+                    "panels": npan, "row_segments": nseg, "ms_per_step": round(ms, 4),
+                    "GBps": round(B / ms / 1e6, 1)})
+        log("panel sweep: %s" % out[-1])
+        M.close()
+        del M
+        torch.cuda.empty_cache()
+    return out
+
+
 def host_cpu_info():
     """(ranks to use, description): one single-threaded rank per core of the
     CPU share this process may run on, capped at the 16 cores a one-GPU job
@@ -498,7 +521,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--forms", default="all", help="all | none | comma list")
+    ap.add_argument("--forms", default="all",
+                    help="all | none | comma list (panel_sweep runs only when listed)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--xstage", type=int, default=None,
@@ -569,6 +593,9 @@ def main():
     if "lowprec" in want:
         log("form lowprec ...")
         forms["lowprec"] = form_lowprec(steps_f, zipf=(A, x))
+    if "panel_sweep" in want:
+        log("form panel_sweep ...")
+        forms["panel_sweep"] = form_panel_sweep(A, x, steps_f)
     del A
     torch.cuda.empty_cache()
     if "laplacian2d_f64" in want:
