@@ -413,17 +413,23 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
       } else if constexpr (BUFX) {
         const rsrc_t rx = make_rsrc(xj, a.xbytes);
         if constexpr (sizeof(TA) == 8 && T > 4) {
-          // 8-byte sums over 8 or more entries per lane: one gather in
-          // flight per entry slot. Holding all T gathered values costs T
-          // more register pairs, which spill at 7 waves or cost a wave (3-D
-          // Laplacian 256^3 FP64: 478 us this way, 492-652 us with all
-          // gathers in flight). Windows of 1024 entries (4 per lane) take
-          // the path below.
+          // 8-byte sums over 8 or more entries per lane: the gathers in two
+          // halves of T / 2, each half all in flight before its products
+          // (every slot: past the window it gathers x[0] into a product the
+          // reduction never reads). All T at once costs T more register
+          // pairs; a predicated gather per slot made each wave wait for one
+          // gather at a time. Measured at 6 waves, against one at a time at
+          // 7: 3-D Laplacian 256^3 FP64 413 -> 392 us, band +-16 FP64 389 ->
+          // 355 us, 2-D Laplacian 1000^2 23.5 -> 21.4 us per call.
+          constexpr int H = T / 2;
 #pragma unroll
-          for (int u = 0; u < T; u++) {
-            const int s = slot(u);
-            if (s >= sh && s < sh + n_s)
-              prod[s] = v[u] * to_acc<TA>(buf_load<TX>(rx, c[u] * sizeof(TX)));
+          for (int h = 0; h < 2; h++) {
+            TA xh[H];
+#pragma unroll
+            for (int i = 0; i < H; i++)
+              xh[i] = to_acc<TA>(buf_load<TX>(rx, c[h * H + i] * sizeof(TX)));
+#pragma unroll
+            for (int i = 0; i < H; i++) prod[slot(h * H + i)] = v[h * H + i] * xh[i];
           }
         } else {
           // Every lane issues all T gathers before storing a product: a
@@ -499,15 +505,21 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
 
 // Waves per SIMD the forward kernel is compiled for: as many workgroups as
 // the 160 KB of LDS per CU holds (one wave per SIMD per 256 lanes), at most
-// 8. Measured on MI355X: 7 waves for FP64 (71 VGPRs, no spills) and 8 for
-// FP32 raise the structured forms by 7-17 % over the compiler's default (4-5
-// waves); trading LDS for an eighth FP64 workgroup spilled registers and
-// lost 30 %.
+// 8. Measured on MI355X: 7-8 waves raise the structured forms by 7-17 %
+// over the compiler's default (4-5 waves); trading LDS for an eighth FP64
+// workgroup spilled registers and lost 30 %.
 template <int WG, int CAP, int MODE, typename TA, typename TI> constexpr int fwd_waves() {
   constexpr int lds = static_cast<int>(sizeof(FwdSmem<WG, CAP, MODE == kRowMap, TA, TI>));
   constexpr int wgs = (160 * 1024) / lds;
   constexpr int w = wgs * (WG / 256);
   return w > 8 ? 8 : (w < 1 ? 1 : w);
+}
+
+template <int WG, int CAP, int MODE, typename TV, typename TA, typename TI>
+constexpr int fwd_kernel_waves() {
+  constexpr int w = fwd_waves<WG, CAP, MODE, TA, TI>() -
+                    ((sizeof(TV) == 2 && MODE == kPlain) || MODE == kPerm ? 1 : 0);
+  return sizeof(TA) == 8 && CAP / WG >= 8 && w > 6 ? 6 : w;
 }
 
 template <int WG, int CAP, int MODE, bool BUFX, typename TV, typename TX,
@@ -518,7 +530,9 @@ __global__ __launch_bounds__(WG)
 // The panel kernel's LDS already holds it at 7 waves, where it does not.
 // Permuted values take one wave fewer too: their T unconditional value
 // loads spilled at 8; band +-16 FP32 adjoint 563 -> 471 us.)
-__attribute__((amdgpu_waves_per_eu(fwd_waves<WG, CAP, MODE, TA, TI>() - ((sizeof(TV) == 2 && MODE == kPlain) || MODE == kPerm ? 1 : 0), 8)))
+// (FP64 sums with 8 or more entries per lane run at most 6 waves: their
+// gathers in halves of four need the registers.)
+__attribute__((amdgpu_waves_per_eu(fwd_kernel_waves<WG, CAP, MODE, TV, TA, TI>(), 8)))
 void k_spmv_fwd(FwdArgs a) {
   __shared__ FwdSmem<WG, CAP, MODE == kRowMap, TA, TI> sm;
   for (int64_t wi = blockIdx.x; wi < a.per_xcd * kXcds; wi += a.grid) {
