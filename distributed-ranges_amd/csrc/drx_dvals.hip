@@ -34,6 +34,7 @@ struct DvalsArgs {
   int64_t lddy, k;
   void *dvals;
   double alpha, beta;
+  uint32_t xbytes;  // bytes of one x column (buffer gathers)
 };
 
 // Row index (within the block) of every window entry, written by lane
@@ -58,17 +59,36 @@ __device__ __forceinline__ void fill_rowid(const DvalsArgs &a, const int *rp_s,
   }
 }
 
+// alpha * acc + beta * old, with one explicit rounding order wherever dvals
+// are written (so grouped and single-entry stores give the same bits).
+template <typename TA> __device__ __forceinline__ TA dv_out(TA alpha, TA acc, TA beta, TA old) {
+  const TA o = alpha * acc;
+  return beta != TA(0) ? __builtin_fma(beta, old, o) : o;
+}
+
 // One row block per workgroup, like the forward kernel: the window's column
-// indices are streamed (coalesced, non-temporal) and x gathered into
-// registers, each entry's row is looked up in LDS (rowid), dy of the
-// block's rows is staged in LDS, and dvals is written with coalesced stores.
-// A last row longer than the window is done by the whole workgroup.
-// (Staging x windows in LDS measured slower here: band +-16 FP32 540 vs
-// 431 us.)
-template <int WG, int CAP, typename TX, typename TD, typename TA, typename TI>
+// indices are streamed (coalesced, non-temporal), x gathered into registers,
+// each entry's row looked up in LDS (rowid), dy of the block's rows staged
+// in LDS, and dvals written with coalesced stores. A last row longer than
+// the window is done by the whole workgroup. (Staging x windows in LDS
+// measured slower here: band +-16 FP32 540 vs 431 us.)
+//
+// BUFX: x gathered through a buffer resource, every lane's T gathers issued
+// before the first product (past the window a lane gathers x[0], and entries
+// outside the block are never stored); a gather inside the per-entry
+// predicate made each wave wait for one gather at a time. VW = 4 (FP32
+// dvals, 32-bit indices, colind and dvals aligned to a group): each lane
+// streams and stores groups of four consecutive entries with 16-byte
+// accesses, slot s holding entry e0 - sh + s as in the forward kernel.
+template <int WG, int CAP, typename TX, typename TD, typename TA, typename TI, bool BUFX = false,
+          int VW = 1>
 __global__ __launch_bounds__(WG) void k_dvals(DvalsArgs a) {
+  static_assert(VW == 1 || (VW == 4 && sizeof(TI) == 4 && sizeof(TD) == 4),
+                "groups of four: 32-bit indices and 4-byte dvals");
   constexpr int T = CAP / WG;
-  constexpr int ROWCAP = CAP / 2;
+  // (8 rows short of CAP / 2: a 4096-entry window's LDS then fits five
+  // workgroups per CU, not four)
+  constexpr int ROWCAP = CAP / 2 - 8;
   __shared__ int rp_s[ROWCAP + 1];
   __shared__ int rowid[CAP];
   __shared__ TA dy_s[ROWCAP];
@@ -88,8 +108,25 @@ __global__ __launch_bounds__(WG) void k_dvals(DvalsArgs a) {
     const int n_s = static_cast<int>(b.es - b.e0);
     const int64_t nr = (b.es == e1 ? r1 : r1 - 1) - b.r0;
     const bool rows_lds = nr < ROWCAP;
+    // entry of slot u of this lane, relative to e0 (in [0, n_s) inside the block)
+    const int sh = VW > 1 ? static_cast<int>(b.e0 & (VW - 1)) : 0;
+    auto entry = [&](int u) -> int {
+      if constexpr (VW == 1)
+        return tid + u * WG;
+      else
+        return (tid + (u / VW) * WG) * VW + (u % VW) - sh;
+    };
     uint32_t c[T];
-    {
+    if constexpr (VW > 1) {
+      const uint32_t ngrp = static_cast<uint32_t>(n_s + sh + VW - 1) / VW;
+      const rsrc_t rc = make_rsrc(colind + (b.e0 - sh), ngrp * VW * 4);
+#pragma unroll
+      for (int g = 0; g < T / VW; g++) {
+        const Words<VW> cw = buf_load<Words<VW>, kNT>(rc, (tid + g * WG) * (VW * 4));
+#pragma unroll
+        for (int i = 0; i < VW; i++) c[g * VW + i] = cw.w[i];
+      }
+    } else {
       const rsrc_t rc = make_rsrc(colind + b.e0, n_s * sizeof(TI));
 #pragma unroll
       for (int u = 0; u < T; u++)
@@ -121,27 +158,74 @@ __global__ __launch_bounds__(WG) void k_dvals(DvalsArgs a) {
         for (int i = tid; i < nr; i += WG) dy_s[i] = to_acc<TA>(dyj[i]);
       }
       __syncthreads();
-      // (Issuing all x reads before the products, as the forward and fused
-      // kernels do, needs 2T more registers here: one wave fewer, band
-      // +-16 FP32 430 -> 461 us.)
+      if constexpr (BUFX) {
+        const rsrc_t rx = make_rsrc(xj, a.xbytes);
+        TA xg[T];
 #pragma unroll
-      for (int u = 0; u < T; u++) {
-        const int idx = tid + u * WG;
-        if (idx < n_s) {
-          const int ri = rowid[idx];
-          const TA d = rows_lds ? dy_s[ri] : to_acc<TA>(dyj[ri]);
-          acc[u] += d * to_acc<TA>(xj[c[u]]);
+        for (int u = 0; u < T; u++) xg[u] = to_acc<TA>(buf_load<TX>(rx, c[u] * sizeof(TX)));
+#pragma unroll
+        for (int u = 0; u < T; u++) {
+          const int q = entry(u);
+          if (q >= 0 && q < n_s) {
+            const int ri = rowid[q];
+            const TA d = rows_lds ? dy_s[ri] : to_acc<TA>(dyj[ri]);
+            acc[u] += d * xg[u];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < T; u++) {
+          const int q = entry(u);
+          if (q >= 0 && q < n_s) {
+            const int ri = rowid[q];
+            const TA d = rows_lds ? dy_s[ri] : to_acc<TA>(dyj[ri]);
+            acc[u] += d * to_acc<TA>(xj[c[u]]);
+          }
         }
       }
       __syncthreads();
     }
+    if constexpr (VW > 1) {
+      // a group inside the block: one 16-byte store; the first and last
+      // groups of the block, entry by entry
 #pragma unroll
-    for (int u = 0; u < T; u++) {
-      const int idx = tid + u * WG;
-      if (idx < n_s) {
-        TA out = alpha * acc[u];
-        if (beta != TA(0)) out += beta * static_cast<TA>(dv[b.e0 + idx]);
-        dv[b.e0 + idx] = static_cast<TD>(out);
+      for (int g = 0; g < T / VW; g++) {
+        const int q0 = (tid + g * WG) * VW - sh;
+        if (q0 >= 0 && q0 + VW <= n_s) {
+          TD *p = dv + b.e0 + q0;
+          Words<VW> o;
+          if (beta != TA(0)) {
+            const Words<VW> old = *reinterpret_cast<const Words<VW> *>(p);
+#pragma unroll
+            for (int i = 0; i < VW; i++)
+              o.w[i] = __builtin_bit_cast(uint32_t, static_cast<TD>(dv_out<TA>(
+                  alpha, acc[g * VW + i], beta,
+                  static_cast<TA>(__builtin_bit_cast(TD, old.w[i])))));
+          } else {
+#pragma unroll
+            for (int i = 0; i < VW; i++)
+              o.w[i] = __builtin_bit_cast(uint32_t, static_cast<TD>(alpha * acc[g * VW + i]));
+          }
+          *reinterpret_cast<Words<VW> *>(p) = o;
+        } else {
+#pragma unroll
+          for (int i = 0; i < VW; i++) {
+            const int q = q0 + i;
+            if (q >= 0 && q < n_s) {
+              const TA old = beta != TA(0) ? static_cast<TA>(dv[b.e0 + q]) : TA(0);
+              dv[b.e0 + q] = static_cast<TD>(dv_out<TA>(alpha, acc[g * VW + i], beta, old));
+            }
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < T; u++) {
+        const int idx = tid + u * WG;
+        if (idx < n_s) {
+          const TA old = beta != TA(0) ? static_cast<TA>(dv[b.e0 + idx]) : TA(0);
+          dv[b.e0 + idx] = static_cast<TD>(dv_out<TA>(alpha, acc[u], beta, old));
+        }
       }
     }
     if (b.es < e1) {
@@ -151,9 +235,8 @@ __global__ __launch_bounds__(WG) void k_dvals(DvalsArgs a) {
         TA s = 0;
         for (int64_t j = 0; j < a.k; j++)
           s += to_acc<TA>(dy[last + j * a.lddy]) * to_acc<TA>(x[col + j * a.ldx]);
-        TA out = alpha * s;
-        if (beta != TA(0)) out += beta * static_cast<TA>(dv[q]);
-        dv[q] = static_cast<TD>(out);
+        const TA old = beta != TA(0) ? static_cast<TA>(dv[q]) : TA(0);
+        dv[q] = static_cast<TD>(dv_out<TA>(alpha, s, beta, old));
       }
     }
   }
@@ -173,8 +256,12 @@ struct FusedArgs {
   int xstage;  // stage each block's x window in LDS
 };
 
+// Rows [ra, rb) of a block: row offsets (relative to e0) and targets from
+// LDS (rows_lds) or, for blocks with more rows than the LDS holds, from
+// memory.
 template <int G, typename TA, typename TY, typename TI>
-__device__ __forceinline__ void fused_rows(const TA *prod, const TA *xg,
+__device__ __forceinline__ void fused_rows(const TA *prod, const TA *xg, const int *rp_s,
+                                           const TA *t_s, bool rows_lds,
                                            const FusedArgs &a, int64_t ra,
                                            int64_t rb, int64_t e0) {
   const TI *rowptr = static_cast<const TI *>(a.rowptr);
@@ -185,12 +272,21 @@ __device__ __forceinline__ void fused_rows(const TA *prod, const TA *xg,
   const int lane = threadIdx.x & (G - 1);
   constexpr int NG = kWG / G;
   for (int64_t r = ra + threadIdx.x / G; r < rb; r += NG) {
-    const int64_t s = row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r) - e0;
-    const int64_t e = row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r + 1) - e0;
+    int64_t s, e;
+    TA tv;
+    if (rows_lds) {
+      s = rp_s[r - ra];
+      e = rp_s[r - ra + 1];
+      tv = t_s[r - ra];
+    } else {
+      s = row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r) - e0;
+      e = row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r + 1) - e0;
+      tv = to_acc<TA>(t[r]);
+    }
     TA acc = 0;
     for (int64_t q = s + lane; q < e; q += G) acc += prod[q];
     acc = group_sum<G>(acc);
-    const TA rr = acc - to_acc<TA>(t[r]);
+    const TA rr = acc - tv;
     if (lane == 0) {
       y[r] = from_acc<TY>(acc);
       res[r] = from_acc<TY>(rr);
@@ -207,8 +303,14 @@ template <int CAP, typename TV, typename TX, typename TY, typename TA,
           typename TI, bool XS = false>
 __global__ __launch_bounds__(kWG) void k_fused_residual(FusedArgs a) {
   constexpr int T = CAP / kWG;
+  // row offsets and targets of blocks of fewer than CAP / 4 rows, staged
+  // once per block: read per row pass from memory they cost each pass a
+  // round trip
+  constexpr int ROWCAP = CAP / 4;
   __shared__ TA prod[CAP];
   __shared__ TA xg[CAP];
+  __shared__ int rp_s[ROWCAP + 1];
+  __shared__ TA t_s[ROWCAP];
   __shared__ TA red[kWG / 64];
   __shared__ TX xs[XS ? kXsCap : 1];
   __shared__ uint32_t xmm[2 * (kWG / 64)];
@@ -239,6 +341,16 @@ __global__ __launch_bounds__(kWG) void k_fused_residual(FusedArgs a) {
         const uint32_t idx = tid + u * kWG;
         w[u] = to_acc<TA>(buf_load<TV, kNT>(rv, idx * sizeof(TV)));
         c[u] = col32(buf_load<TI, kNT>(rc, idx * sizeof(TI)));
+      }
+    }
+    const int64_t nr = rs_end - b.r0;
+    const bool rows_lds = nr < ROWCAP;
+    if (rows_lds) {
+      const TI *rowptr = static_cast<const TI *>(a.rowptr);
+      const TY *t = static_cast<const TY *>(a.t);
+      for (int i = tid; i <= nr; i += kWG) {
+        rp_s[i] = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, b.r0 + i) - b.e0);
+        if (i < nr) t_s[i] = to_acc<TA>(t[b.r0 + i]);
       }
     }
     if constexpr (XS) {
@@ -275,17 +387,16 @@ __global__ __launch_bounds__(kWG) void k_fused_residual(FusedArgs a) {
       prod[idx] = w[u] * xv[u];
     }
     __syncthreads();
-    const int64_t nr = rs_end - b.r0;
     if (nr > 0) {
       const int64_t avg = n_s / nr;
       if (avg <= 3)
-        fused_rows<1, TA, TY, TI>(prod, xg, a, b.r0, rs_end, b.e0);
+        fused_rows<1, TA, TY, TI>(prod, xg, rp_s, t_s, rows_lds, a, b.r0, rs_end, b.e0);
       else if (avg <= 12)
-        fused_rows<4, TA, TY, TI>(prod, xg, a, b.r0, rs_end, b.e0);
+        fused_rows<4, TA, TY, TI>(prod, xg, rp_s, t_s, rows_lds, a, b.r0, rs_end, b.e0);
       else if (avg <= 48)
-        fused_rows<16, TA, TY, TI>(prod, xg, a, b.r0, rs_end, b.e0);
+        fused_rows<16, TA, TY, TI>(prod, xg, rp_s, t_s, rows_lds, a, b.r0, rs_end, b.e0);
       else
-        fused_rows<64, TA, TY, TI>(prod, xg, a, b.r0, rs_end, b.e0);
+        fused_rows<64, TA, TY, TI>(prod, xg, rp_s, t_s, rows_lds, a, b.r0, rs_end, b.e0);
     }
     __syncthreads();
     if (b.es < e1) {
@@ -340,30 +451,50 @@ void run_dvals(const Plan &p, const void *colind, int xtype, const void *x,
   a.dvals = dvals;
   a.alpha = alpha;
   a.beta = beta;
+  // x gathered through a buffer resource when one column fits 32-bit offsets
+  const uint64_t xb = static_cast<uint64_t>(std::max<int64_t>(p.ncols, 1)) * dtype_size(xtype);
+  const bool bufx = xb < (uint64_t(1) << 32);
+  a.xbytes = bufx ? static_cast<uint32_t>(xb) : 0;
   const dim3 grid(static_cast<unsigned>(a.grid));
   const bool i32 = p.itype == DRX_I32;
 #define DRX_DV_I(WG, CAP, TX, TD, TA)                                          \
   do {                                                                         \
-    if (i32)                                                                   \
+    if (i32 && bufx)                                                           \
+      hipLaunchKernelGGL((k_dvals<WG, CAP, TX, TD, TA, int32_t, true>), grid,  \
+                         dim3(WG), 0, s, a);                                   \
+    else if (i32)                                                              \
       hipLaunchKernelGGL((k_dvals<WG, CAP, TX, TD, TA, int32_t>), grid,        \
+                         dim3(WG), 0, s, a);                                   \
+    else if (bufx)                                                             \
+      hipLaunchKernelGGL((k_dvals<WG, CAP, TX, TD, TA, int64_t, true>), grid,  \
                          dim3(WG), 0, s, a);                                   \
     else                                                                       \
       hipLaunchKernelGGL((k_dvals<WG, CAP, TX, TD, TA, int64_t>), grid,        \
                          dim3(WG), 0, s, a);                                   \
   } while (0)
-  // FP32 with 32-bit indices and rows of >= 16 entries on average: the
-  // plan's 2048-entry blocks, 16 per lane (band +-16: 2575 vs 2331 GB/s; the
-  // 7-entry rows of a 3-D Laplacian run slower that way, 2036 vs 2303)
-  const bool wide = xtype == DRX_F32 && dvtype == DRX_F32 && i32 && p.rec_wide != nullptr &&
-                    tuned_wide() != 0 && p.nnz >= 16 * p.nrows;
+  // FP32 with 32-bit indices and rows of >= 16 entries on average: windows
+  // of 4096 entries, 16 per lane (band +-16: 2575 vs 2331 GB/s with gathers
+  // one at a time; the 7-entry rows of a 3-D Laplacian ran slower that way,
+  // 2036 vs 2303)
+  const bool wide = xtype == DRX_F32 && dvtype == DRX_F32 && i32 && bufx &&
+                    p.rec_wide != nullptr && tuned_wide() != 0 && p.nnz >= 16 * p.nrows;
   if (wide) {
     a.rec = p.rec_wide;
     a.nblocks = p.nblocks_wide;
     const Grid gw = grid_for(p.nblocks_wide);
     a.per_xcd = gw.per_xcd;
     a.grid = gw.grid;
-    hipLaunchKernelGGL((k_dvals<256, 4096, float, float, float, int32_t>),
-                       dim3(static_cast<unsigned>(a.grid)), dim3(256), 0, s, a);
+    // groups of four entries per lane when colind and dvals are aligned to
+    // a group (knob vec 0 or 4)
+    const int vec = tuned_vec();
+    const bool g4 = (vec == 0 || vec == 4) && reinterpret_cast<uintptr_t>(colind) % 16 == 0 &&
+                    reinterpret_cast<uintptr_t>(dvals) % 16 == 0;
+    if (g4)
+      hipLaunchKernelGGL((k_dvals<256, 4096, float, float, float, int32_t, true, 4>),
+                         dim3(static_cast<unsigned>(a.grid)), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_dvals<256, 4096, float, float, float, int32_t, true>),
+                         dim3(static_cast<unsigned>(a.grid)), dim3(256), 0, s, a);
     DRX_HIP(hipGetLastError());
     return;
   }

@@ -249,10 +249,45 @@ def test_adjoint_with_stored_transposed_values(orc, dtype):
     assert componentwise_err(dx2.cpu().numpy(), ref, absr) < (1e-12 if dtype == torch.float64 else 1e-5)
 
 
+@pytest.mark.parametrize("beta", [0.0, -0.75])
+def test_dvals_grouped_entries_bitexact(orc, beta):
+    """FP32 dvals over 4096-entry windows with groups of four entries per
+    lane (16-byte colind loads and dvals stores; vec 0 or 4) give the same
+    bits as one entry per lane (vec 1), with windows starting at every
+    offset inside a group, long and empty rows, and output buffers starting
+    one to three elements into a larger allocation (single entries); beta
+    reads the old values."""
+    rng = np.random.default_rng(51)
+    lens = [0] * 7 + list(rng.integers(16, 70, 4000)) + [9000, 0, 3, 20000, 17, 1]
+    n = 40000
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    ci = np.concatenate([np.sort(rng.choice(n, L, replace=False)) for L in lens]).astype(np.int64)
+    v = rng.uniform(-1, 1, ci.size).astype(np.float32)
+    Ad = drx.from_host(rp, ci, v, (len(lens), n))
+    x = torch.as_tensor(rng.uniform(-1, 1, n).astype(np.float32)).to(DEV)
+    dy = torch.as_tensor(rng.uniform(-1, 1, len(lens)).astype(np.float32)).to(DEV)
+    d0 = torch.as_tensor(rng.uniform(-1, 1, ci.size).astype(np.float32)).to(DEV)
+    outs = []
+    for vec in (1, 4, 0):
+        for off in (0, 1, 3):
+            buf = torch.zeros(ci.size + off, dtype=torch.float32, device=DEV)
+            o = buf[off:]
+            o.copy_(d0)
+            with drx.tuned(vec=vec):
+                Ad.dvals(x, dy, out=o, alpha=1.5, beta=beta)
+            outs.append(o)
+    torch.cuda.synchronize()
+    for o in outs[1:]:
+        assert torch.equal(outs[0], o)
+    ref = 1.5 * orc.dvals(rp, ci, x.cpu().numpy(), dy.cpu().numpy()) + beta * d0.cpu().numpy()
+    assert np.abs(outs[0].cpu().numpy() - ref).max() <= 1e-5
+    Ad.close()
+
+
 @pytest.mark.parametrize("wide", [1, 0])
 def test_dvals_fp32_wide_blocks(orc, wide):
-    """FP32 dvals over rows of >= 16 entries stream the plan's 2048-entry
-    blocks (or the plan's own with wide = 0), with long and empty rows."""
+    """FP32 dvals over rows of >= 16 entries stream 4096-entry windows (or
+    the plan's own with wide = 0), with long and empty rows."""
     rng = np.random.default_rng(40 + wide)
     lens = [0] * 50 + list(rng.integers(10, 60, 3000)) + [9000, 0, 20000]
     n = 30000
