@@ -299,14 +299,21 @@ __device__ __forceinline__ void fused_rows(const TA *prod, const TA *xg, const i
 // LDS; entries read x from there, or from memory past it.
 constexpr uint32_t kXsCap = 512;
 
+// VW = 4 (not with XS; 32-bit indices, four values in at most 16 bytes,
+// vals / colind aligned to a group): lanes stream groups of four
+// consecutive entries with 16-byte loads, slot s of prod / xg holding entry
+// e0 - sh + s as in the forward kernel.
 template <int CAP, typename TV, typename TX, typename TY, typename TA,
-          typename TI, bool XS = false>
+          typename TI, bool XS = false, int VW = 1>
 __global__ __launch_bounds__(kWG) void k_fused_residual(FusedArgs a) {
+  static_assert(VW == 1 || (VW == 4 && !XS && sizeof(TI) == 4 && 4 * sizeof(TV) <= 16),
+                "groups of four: no x staging, 32-bit indices, at most 16 bytes of values");
   constexpr int T = CAP / kWG;
-  // row offsets and targets of blocks of fewer than CAP / 4 rows, staged
-  // once per block: read per row pass from memory they cost each pass a
-  // round trip
-  constexpr int ROWCAP = CAP / 4;
+  // row offsets and targets of blocks of fewer than CAP / 4 - 8 rows,
+  // staged once per block: read per row pass from memory they cost each
+  // pass a round trip (8 rows short, a 2048-entry window's LDS fits eight
+  // workgroups per CU)
+  constexpr int ROWCAP = CAP / 4 - 8;
   __shared__ TA prod[CAP];
   __shared__ TA xg[CAP];
   __shared__ int rp_s[ROWCAP + 1];
@@ -333,7 +340,33 @@ __global__ __launch_bounds__(kWG) void k_fused_residual(FusedArgs a) {
     // time. Slots past n_s read x[0] and store products nothing reads.
     uint32_t c[T];
     TA w[T], xv[T];
-    {
+    const int sh = VW > 1 ? static_cast<int>(b.e0 & (VW - 1)) : 0;
+    auto slot = [&](int u) -> int {
+      if constexpr (VW == 1)
+        return tid + u * kWG;
+      else
+        return (tid + (u / VW) * kWG) * VW + (u % VW);
+    };
+    if constexpr (VW > 1) {
+      const int64_t ea = b.e0 - sh;
+      const uint32_t ngrp = static_cast<uint32_t>(n_s + sh + VW - 1) / VW;
+      constexpr uint32_t VB = VW * sizeof(TV);
+      const rsrc_t rv = make_rsrc(vals + ea, ngrp * VB);
+      const rsrc_t rc = make_rsrc(colind + ea, ngrp * VW * 4);
+#pragma unroll
+      for (int g = 0; g < T / VW; g++) {
+        const uint32_t gi = tid + g * kWG;
+        const Words<VW> cw = buf_load<Words<VW>, kNT>(rc, gi * (VW * 4));
+        const Words<(VB + 3) / 4> vw = buf_load<Words<(VB + 3) / 4>, kNT>(rv, gi * VB);
+        TV t[VW];
+        __builtin_memcpy(t, vw.w, VB);
+#pragma unroll
+        for (int i = 0; i < VW; i++) {
+          c[g * VW + i] = cw.w[i];
+          w[g * VW + i] = to_acc<TA>(t[i]);
+        }
+      }
+    } else {
       const rsrc_t rv = make_rsrc(vals + b.e0, n_s * sizeof(TV));
       const rsrc_t rc = make_rsrc(colind + b.e0, n_s * sizeof(TI));
 #pragma unroll
@@ -382,21 +415,21 @@ __global__ __launch_bounds__(kWG) void k_fused_residual(FusedArgs a) {
     }
 #pragma unroll
     for (int u = 0; u < T; u++) {
-      const int idx = tid + u * kWG;
-      xg[idx] = xv[u];
-      prod[idx] = w[u] * xv[u];
+      xg[slot(u)] = xv[u];
+      prod[slot(u)] = w[u] * xv[u];
     }
     __syncthreads();
     if (nr > 0) {
       const int64_t avg = n_s / nr;
+      const TA *pr = prod + sh, *xq = xg + sh;
       if (avg <= 3)
-        fused_rows<1, TA, TY, TI>(prod, xg, rp_s, t_s, rows_lds, a, b.r0, rs_end, b.e0);
+        fused_rows<1, TA, TY, TI>(pr, xq, rp_s, t_s, rows_lds, a, b.r0, rs_end, b.e0);
       else if (avg <= 12)
-        fused_rows<4, TA, TY, TI>(prod, xg, rp_s, t_s, rows_lds, a, b.r0, rs_end, b.e0);
+        fused_rows<4, TA, TY, TI>(pr, xq, rp_s, t_s, rows_lds, a, b.r0, rs_end, b.e0);
       else if (avg <= 48)
-        fused_rows<16, TA, TY, TI>(prod, xg, rp_s, t_s, rows_lds, a, b.r0, rs_end, b.e0);
+        fused_rows<16, TA, TY, TI>(pr, xq, rp_s, t_s, rows_lds, a, b.r0, rs_end, b.e0);
       else
-        fused_rows<64, TA, TY, TI>(prod, xg, rp_s, t_s, rows_lds, a, b.r0, rs_end, b.e0);
+        fused_rows<64, TA, TY, TI>(pr, xq, rp_s, t_s, rows_lds, a, b.r0, rs_end, b.e0);
     }
     __syncthreads();
     if (b.es < e1) {
@@ -558,8 +591,25 @@ void run_fused(const Plan &p, int vtype, const void *colind, const void *vals,
   const dim3 grid(static_cast<unsigned>(a.grid)), block(kWG);
   const bool i32 = p.itype == DRX_I32;
   const bool small = p.cap == 1024;
+  // groups of four entries per lane: 32-bit indices, vals / colind aligned
+  // to a group, no x staging (knob vec 0 or 4)
+  const int vec = tuned_vec();
+  const bool g4 = i32 && !a.xstage && (vec == 0 || vec == 4) && dtype_size(vtype) <= 4 &&
+                  reinterpret_cast<uintptr_t>(colind) % 16 == 0 &&
+                  reinterpret_cast<uintptr_t>(vals) % (4 * dtype_size(vtype)) == 0;
 #define DRX_FU_X(TV, TX, TY, TA, XS)                                           \
   do {                                                                         \
+    if constexpr (!(XS) && 4 * sizeof(TV) <= 16) {                             \
+      if (g4) {                                                                \
+        if (small)                                                             \
+          hipLaunchKernelGGL((k_fused_residual<1024, TV, TX, TY, TA, int32_t, false, 4>), \
+                             grid, block, 0, s, a);                            \
+        else                                                                   \
+          hipLaunchKernelGGL((k_fused_residual<2048, TV, TX, TY, TA, int32_t, false, 4>), \
+                             grid, block, 0, s, a);                            \
+        break;                                                                 \
+      }                                                                        \
+    }                                                                          \
     if (i32 && small)                                                          \
       hipLaunchKernelGGL((k_fused_residual<1024, TV, TX, TY, TA, int32_t, XS>), \
                          grid, block, 0, s, a);                                \

@@ -135,12 +135,36 @@ def test_fused_residual_matches_separate(orc, dtype):
     assert np.abs(dv.cpu().numpy() - ref).max() <= (1e-15 if dtype == torch.float64 else 1e-6) * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("vdt", [torch.bfloat16, torch.float16])
+def test_fused_residual_16bit_storage(orc, vdt):
+    """16-bit vals and x, FP32 sums, y / r / dvals in FP32: against the
+    oracle on the dequantized inputs, groups of four entries per lane and
+    single entries bit for bit."""
+    A = drx.gen.band(50000, 16, 16, vdtype=torch.float32, seed=41)
+    A.vals = A.vals.to(vdt)
+    x = drx.gen.uniform(A.shape[1], torch.float32, -1, 1, seed=42).to(vdt)
+    t = drx.gen.uniform(A.shape[0], torch.float32, -1, 1, seed=43)
+    y, r, dv = A.fused_residual(x, t)
+    with drx.tuned(vec=1):
+        y1, r1, dv1 = A.fused_residual(x, t)
+    torch.cuda.synchronize()
+    assert torch.equal(y, y1) and torch.equal(r, r1) and torch.equal(dv, dv1)
+    rp, ci = A.rowptr.cpu().numpy(), A.colind.cpu().numpy()
+    v64, x64 = A.vals.double().cpu().numpy(), x.double().cpu().numpy()
+    ystar, absy = orc.spmv(rp, ci, v64, x64)
+    assert componentwise_err(y.cpu().numpy(), ystar, absy) < 1e-5
+    assert torch.equal(r, y - t)
+    ref = orc.dvals(rp, ci, x64, r.double().cpu().numpy())
+    assert np.abs(dv.double().cpu().numpy() - ref).max() <= 1e-6 * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("kind", ["band", "wide_band", "random"])
 def test_fused_residual_x_staging_bitexact(orc, dtype, kind):
     """xstage = 2 (each block's x window staged in LDS; windows wider than
-    the staging buffer read the rest from memory) and the default give the
-    same bits as per-entry gathers (xstage = 0)."""
+    the staging buffer read the rest from memory), the default, groups of
+    four entries per lane and unaligned vals give the same bits as per-entry
+    gathers (xstage = 0)."""
     if kind == "band":
         A = drx.gen.band(60000, 16, 16, vdtype=dtype, seed=31)
     elif kind == "wide_band":
@@ -151,10 +175,19 @@ def test_fused_residual_x_staging_bitexact(orc, dtype, kind):
     t = drx.gen.uniform(A.shape[0], dtype, -1, 1, seed=35)
     outs = {}
     for xs in (0, 2, 1):
-        with drx.tuned(xstage=xs):
+        with drx.tuned(xstage=xs, vec=1):
             outs[xs] = A.fused_residual(x, t)
+    # groups of four entries per lane (FP32, default vec) and single entries
+    # when vals start one element into a larger buffer
+    outs["g4"] = A.fused_residual(x, t)
+    buf = torch.zeros(A.nnz + 1, dtype=dtype, device=DEV)
+    buf[1:] = A.vals
+    v0 = A.vals
+    A.vals = buf[1:]
+    outs["off"] = A.fused_residual(x, t)
+    A.vals = v0
     torch.cuda.synchronize()
-    for xs in (2, 1):
+    for xs in (2, 1, "g4", "off"):
         for a, b in zip(outs[0], outs[xs]):
             assert torch.equal(a, b), (kind, xs)
     rp, ci, v = A.rowptr.cpu().numpy(), A.colind.cpu().numpy(), A.vals.cpu().numpy()
