@@ -22,8 +22,11 @@
 #include <concepts>
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
+#include <iterator>
 #include <memory>
 #include <ranges>
+#include <thread>
 #include <vector>
 
 #include "dr/detail/drx.hpp"
@@ -87,6 +90,32 @@ void upload_share(const T *vals, const I *colind, const I *rowptr, std::size_t n
   v.upload(vals, nnz, stream());
   c.upload(colind, nnz, stream());
   rp.upload(rowptr, rows, stream());
+}
+
+// Copy n partial values into res at out: large copies into contiguous
+// memory run on up to 8 threads (a single thread copies a 64 MB partial at
+// about 20 GB/s, which was a third of the host part of a gemv call).
+template <typename T, typename It> void copy_out(const T *src, std::size_t n, It out) {
+  constexpr std::size_t kPar = std::size_t(8) << 20;  // bytes
+  if constexpr (std::contiguous_iterator<It>) {
+    const std::size_t bytes = n * sizeof(T);
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const std::size_t nt = std::min<std::size_t>({8, hw, bytes / kPar});
+    if (nt >= 2) {
+      T *dst = std::to_address(out);
+      const std::size_t per = (n + nt - 1) / nt;
+      std::vector<std::thread> th;
+      th.reserve(nt - 1);
+      for (std::size_t t = 1; t < nt; t++) {
+        const std::size_t b0 = std::min(n, t * per), b1 = std::min(n, b0 + per);
+        th.emplace_back([=] { std::memcpy(dst + b0, src + b0, (b1 - b0) * sizeof(T)); });
+      }
+      std::memcpy(dst, src, std::min(n, per) * sizeof(T));
+      for (auto &x : th) x.join();
+      return;
+    }
+  }
+  std::copy(src, src + n, out);
 }
 
 }  // namespace __detail
@@ -189,7 +218,7 @@ private:
       const std::size_t css = std::min(segment_size_, m - j * segment_size_);
       for (std::size_t i = 0; i < k; i++) {
         const T *piece = static_cast<const T *>(parts[j]) + i * segment_size_;
-        std::copy(piece, piece + css, out + m * i + j * segment_size_);
+        __detail::copy_out(piece, css, out + m * i + j * segment_size_);
       }
     }
   }
@@ -372,7 +401,7 @@ private:
         const T *piece = static_cast<const T *>(parts[i]) + kk * row_sizes_[i];
         if (written <= first) std::fill(col + written, col + first + 1, T(0));
         col[first] += piece[0];
-        std::copy(piece + 1, piece + (last - first), col + first + 1);
+        __detail::copy_out(piece + 1, last - first - 1, col + first + 1);
         written = std::max(written, last);
       }
       if (written < m) std::fill(col + written, col + m, T(0));
