@@ -652,13 +652,85 @@ __device__ __forceinline__ void reduce_rows_mrhs(
   }
 }
 
+// Blocks of fewer long rows than waves (rows of a wide band: one row of
+// ~2000 entries per block): every row by the whole workgroup, each lane
+// walking its entries with two x rows in flight, the KP sums reduced
+// across the waves through LDS. With a wave per row the other waves sat
+// idle and each lane walked four times as many entries one after another.
+template <int WG, int KP, typename TX, typename TY, typename TA, typename TI>
+__device__ __forceinline__ void reduce_rows_mrhs_wg(
+    const TA *v_s, const uint32_t *c_s, const int *rp_s, bool rows_lds, const FwdArgs &a,
+    int64_t r0, int64_t e0, int64_t nr, const TX *xrc, int64_t j0, int kk, TA alpha, TA beta,
+    TA *red) {
+  constexpr int NW = WG / 64;
+  static_assert(KP <= 16, "red holds 16 sums per wave");
+  const TI *rowptr = static_cast<const TI *>(a.rowptr);
+  TY *y = static_cast<TY *>(a.y);
+  const int tid = threadIdx.x;
+  for (int64_t i = 0; i < nr; i++) {
+    int s, e;
+    if (rows_lds) {
+      s = rp_s[i];
+      e = rp_s[i + 1];
+    } else {
+      s = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r0 + i) - e0);
+      e = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r0 + i + 1) - e0);
+    }
+    TA acc[KP];
+#pragma unroll
+    for (int jj = 0; jj < KP; jj++) acc[jj] = 0;
+    int q = s + tid;
+    for (; q + WG < e; q += 2 * WG) {
+      TA x0[KP], x1[KP];
+      load_xrow<KP>(xrc + static_cast<size_t>(c_s[q]) * KP, x0);
+      load_xrow<KP>(xrc + static_cast<size_t>(c_s[q + WG]) * KP, x1);
+      const TA v0 = v_s[q], v1 = v_s[q + WG];
+#pragma unroll
+      for (int jj = 0; jj < KP; jj++) {
+        acc[jj] += v0 * x0[jj];
+        acc[jj] += v1 * x1[jj];
+      }
+    }
+    if (q < e) {
+      TA x0[KP];
+      load_xrow<KP>(xrc + static_cast<size_t>(c_s[q]) * KP, x0);
+      const TA v0 = v_s[q];
+#pragma unroll
+      for (int jj = 0; jj < KP; jj++) acc[jj] += v0 * x0[jj];
+    }
+#pragma unroll
+    for (int jj = 0; jj < KP; jj++) acc[jj] = group_sum<64>(acc[jj]);
+    __syncthreads();
+    if ((tid & 63) == 0) {
+#pragma unroll
+      for (int jj = 0; jj < KP; jj++) red[jj * NW + (tid >> 6)] = acc[jj];
+    }
+    __syncthreads();
+    if (tid < kk) {
+      TA sum = 0;
+#pragma unroll
+      for (int w = 0; w < NW; w++) sum += red[tid * NW + w];
+      TY *yp = y + (j0 + tid) * a.ldy + r0 + i;
+      TA out = alpha * sum;
+      if (beta != TA(0)) out += beta * to_acc<TA>(*yp);
+      *yp = from_acc<TY>(out);
+    }
+  }
+  __syncthreads();  // red is reused by the next chunk of right-hand sides
+}
+
 template <int WG, int KP, typename TX, typename TY, typename TA, typename TI>
 __device__ __forceinline__ void reduce_rows_mrhs_any(
     int n_s, const TA *v_s, const uint32_t *c_s, const int *rp_s,
     bool rows_lds, const FwdArgs &a, int64_t r0, int64_t e0, int64_t nr,
-    const TX *xrc, int64_t j0, int kk, TA alpha, TA beta) {
+    const TX *xrc, int64_t j0, int kk, TA alpha, TA beta, TA *red) {
   if (nr <= 0) return;
   const int64_t avg = (n_s / nr) >> a.gshift;
+  if (nr < WG / 64 && n_s / nr > 96) {
+    reduce_rows_mrhs_wg<WG, KP, TX, TY, TA, TI>(v_s, c_s, rp_s, rows_lds, a, r0, e0, nr, xrc, j0,
+                                               kk, alpha, beta, red);
+    return;
+  }
 #define DRX_RM(G) reduce_rows_mrhs<WG, G, KP, TX, TY, TA, TI>(v_s, c_s, rp_s, rows_lds, a, r0, e0, nr, xrc, j0, kk, alpha, beta)
   if (avg <= 3)
     DRX_RM(1);
@@ -748,7 +820,7 @@ __device__ __forceinline__ void process_block_mrhs(const FwdArgs &a,
     const int kk = static_cast<int>(a.k - jc * KP < KP ? a.k - jc * KP : KP);
     reduce_rows_mrhs_any<WG, KP, TX, TY, TA, TI>(
         n_s, sm.v, sm.c, sm.rp_s, rows_lds, a, b.r0, b.e0, nr,
-        xr + jc * a.ncols * KP, jc * KP, kk, alpha, beta);
+        xr + jc * a.ncols * KP, jc * KP, kk, alpha, beta, sm.red);
   }
 
   if (b.es < e1) {
