@@ -157,7 +157,9 @@ static const void *batch_kernel(int vw) {
 // k >= 2: the same walk over (item, block) with the multi-RHS block body.
 template <int WG, int CAP, int KP, typename TV, typename TX, typename TY,
           typename TA, typename TI>
-__global__ __launch_bounds__(WG) void k_batch_mrhs(const FwdArgs *items,
+__global__ __launch_bounds__(WG)
+__attribute__((amdgpu_waves_per_eu(mrhs_waves<WG, CAP, KP, TX, TA>(), 8)))
+void k_batch_mrhs(const FwdArgs *items,
                                                    const int *modes,
                                                    const int64_t *off,
                                                    int64_t nitems, int64_t total) {

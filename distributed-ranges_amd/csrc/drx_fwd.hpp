@@ -602,11 +602,11 @@ template <int WG, int G, int KP, typename TX, typename TY, typename TA,
 __device__ __forceinline__ void reduce_rows_mrhs(
     const TA *v_s, const uint32_t *c_s, const int *rp_s, bool rows_lds,
     const FwdArgs &a, int64_t r0, int64_t e0, int64_t nr, const TX *xrc,
-    int64_t j0, int kk, TA alpha, TA beta) {
+    int64_t j0, int kk, TA alpha, TA beta, int tid) {
   const TI *rowptr = static_cast<const TI *>(a.rowptr);
   TY *y = static_cast<TY *>(a.y);
-  const int lane = threadIdx.x & (G - 1);
-  for (int64_t i = threadIdx.x / G; i < nr; i += WG / G) {
+  const int lane = tid & (G - 1);
+  for (int64_t i = tid / G; i < nr; i += WG / G) {
     int s, e;
     if (rows_lds) {
       s = rp_s[i];
@@ -652,6 +652,154 @@ __device__ __forceinline__ void reduce_rows_mrhs(
   }
 }
 
+// Wide chunks (an x row of KP values spans S = KP / KL 16-byte pieces, KL =
+// 16 / sizeof(x)): S lanes share an entry, each loading and summing one
+// 16-byte piece, so a lane holds KL sums instead of KP and keeps four
+// entries' pieces in flight. Groups of G * S lanes take a row (G entry
+// lanes), and the pieces' sums are added across the G entry lanes with
+// lane shuffles (stride S). Holding the whole KP-vector per lane (16 sums
+// and 16 values per entry in flight for FP32 k = 16) limited the batched
+// kernel to two waves per SIMD with two entries in flight per lane.
+template <int WG, int G, int KP, typename TX, typename TY, typename TA,
+          typename TI>
+__device__ __forceinline__ void reduce_rows_mrhs_split(
+    const TA *v_s, const uint32_t *c_s, const int *rp_s, bool rows_lds,
+    const FwdArgs &a, int64_t r0, int64_t e0, int64_t nr, const TX *xrc,
+    int64_t j0, int kk, TA alpha, TA beta, int tid) {
+  constexpr int KL = 16 / static_cast<int>(sizeof(TX));
+  constexpr int S = KP / KL;
+  constexpr int GS = G * S;
+  static_assert(S >= 2 && GS <= 64 && KP % KL == 0, "pieces of 16 bytes within a wave");
+  const TI *rowptr = static_cast<const TI *>(a.rowptr);
+  TY *y = static_cast<TY *>(a.y);
+  const int lg = tid & (GS - 1);
+  const int sub = lg % S, el = lg / S;
+  for (int64_t i = tid / GS; i < nr; i += WG / GS) {
+    int s, e;
+    if (rows_lds) {
+      s = rp_s[i];
+      e = rp_s[i + 1];
+    } else {
+      s = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r0 + i) - e0);
+      e = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r0 + i + 1) - e0);
+    }
+    TA acc[KL];
+#pragma unroll
+    for (int j = 0; j < KL; j++) acc[j] = 0;
+    const TX *xs = xrc + sub * KL;
+    int q = s + el;
+    for (; q + 3 * G < e; q += 4 * G) {
+      TA x0[KL], x1[KL], x2[KL], x3[KL];
+      load_xrow<KL>(xs + static_cast<size_t>(c_s[q]) * KP, x0);
+      load_xrow<KL>(xs + static_cast<size_t>(c_s[q + G]) * KP, x1);
+      load_xrow<KL>(xs + static_cast<size_t>(c_s[q + 2 * G]) * KP, x2);
+      load_xrow<KL>(xs + static_cast<size_t>(c_s[q + 3 * G]) * KP, x3);
+      const TA v0 = v_s[q], v1 = v_s[q + G], v2 = v_s[q + 2 * G], v3 = v_s[q + 3 * G];
+#pragma unroll
+      for (int j = 0; j < KL; j++) {
+        acc[j] += v0 * x0[j];
+        acc[j] += v1 * x1[j];
+        acc[j] += v2 * x2[j];
+        acc[j] += v3 * x3[j];
+      }
+    }
+    for (; q < e; q += G) {
+      TA x0[KL];
+      load_xrow<KL>(xs + static_cast<size_t>(c_s[q]) * KP, x0);
+      const TA v0 = v_s[q];
+#pragma unroll
+      for (int j = 0; j < KL; j++) acc[j] += v0 * x0[j];
+    }
+#pragma unroll
+    for (int o = S; o < GS; o *= 2) {
+#pragma unroll
+      for (int j = 0; j < KL; j++) acc[j] += __shfl_xor(acc[j], o, 64);
+    }
+    if (el == 0) {
+      const int64_t row = r0 + i;
+#pragma unroll
+      for (int j = 0; j < KL; j++) {
+        const int col = sub * KL + j;
+        if (col < kk) {
+          TY *yp = y + (j0 + col) * a.ldy + row;
+          TA out = alpha * acc[j];
+          if (beta != TA(0)) out += beta * to_acc<TA>(*yp);
+          *yp = from_acc<TY>(out);
+        }
+      }
+    }
+  }
+}
+
+// Sum of row entries [s, e) (LDS window, or memory for the long last row)
+// times their x rows, by the whole workgroup, into red: lanes take 16-byte
+// pieces of the x rows when KP spans two or more (S lanes per entry), else
+// whole rows; red[c * NW + w] holds wave w's sum for column c of the chunk.
+template <int WG, int KP, int MODE, typename TV, typename TX, typename TA, typename TI, bool LDSW>
+__device__ __forceinline__ void wg_row_sums(const FwdArgs &a, const TA *v_s, const uint32_t *c_s,
+                                            int64_t s, int64_t e, const TX *xrc, TA *red,
+                                            int tid) {
+  constexpr int NW = WG / 64;
+  constexpr int KL0 = 16 / static_cast<int>(sizeof(TX));
+  constexpr bool kSplit = (sizeof(TX) == 4 || sizeof(TX) == 8) && KP >= 2 * KL0;
+  constexpr int KL = kSplit ? KL0 : KP;
+  constexpr int S = KP / KL;
+  const int sub = tid % S, el = tid / S;
+  constexpr int NE = WG / S;  // entry lanes
+  const TI *colind = static_cast<const TI *>(a.colind);
+  const TV *vals = static_cast<const TV *>(a.vals);
+  const TI *perm = static_cast<const TI *>(a.perm);
+  auto entry = [&](int64_t q, TA &w, uint32_t &c) {
+    if constexpr (LDSW) {
+      w = v_s[q];
+      c = c_s[q];
+    } else {
+      w = to_acc<TA>(long_val<MODE>(vals, perm, q));
+      c = col32(ld_stream(colind + q));
+    }
+  };
+  const TX *xs = xrc + sub * KL;
+  TA acc[KL];
+#pragma unroll
+  for (int j = 0; j < KL; j++) acc[j] = 0;
+  int64_t q = s + el;
+  for (; q + NE < e; q += 2 * NE) {
+    TA w0, w1;
+    uint32_t c0, c1;
+    entry(q, w0, c0);
+    entry(q + NE, w1, c1);
+    TA x0[KL], x1[KL];
+    load_xrow<KL>(xs + static_cast<size_t>(c0) * KP, x0);
+    load_xrow<KL>(xs + static_cast<size_t>(c1) * KP, x1);
+#pragma unroll
+    for (int j = 0; j < KL; j++) {
+      acc[j] += w0 * x0[j];
+      acc[j] += w1 * x1[j];
+    }
+  }
+  if (q < e) {
+    TA w0;
+    uint32_t c0;
+    entry(q, w0, c0);
+    TA x0[KL];
+    load_xrow<KL>(xs + static_cast<size_t>(c0) * KP, x0);
+#pragma unroll
+    for (int j = 0; j < KL; j++) acc[j] += w0 * x0[j];
+  }
+  // across the wave's entry lanes (stride S), then per wave into red
+#pragma unroll
+  for (int o = S; o < 64; o *= 2) {
+#pragma unroll
+    for (int j = 0; j < KL; j++) acc[j] += __shfl_xor(acc[j], o, 64);
+  }
+  __syncthreads();
+  if ((tid & 63) < S) {
+#pragma unroll
+    for (int j = 0; j < KL; j++) red[(sub * KL + j) * NW + (tid >> 6)] = acc[j];
+  }
+  __syncthreads();
+}
+
 // Blocks of fewer long rows than waves (rows of a wide band: one row of
 // ~2000 entries per block): every row by the whole workgroup, each lane
 // walking its entries with two x rows in flight, the KP sums reduced
@@ -661,12 +809,11 @@ template <int WG, int KP, typename TX, typename TY, typename TA, typename TI>
 __device__ __forceinline__ void reduce_rows_mrhs_wg(
     const TA *v_s, const uint32_t *c_s, const int *rp_s, bool rows_lds, const FwdArgs &a,
     int64_t r0, int64_t e0, int64_t nr, const TX *xrc, int64_t j0, int kk, TA alpha, TA beta,
-    TA *red) {
+    TA *red, int tid) {
   constexpr int NW = WG / 64;
   static_assert(KP <= 16, "red holds 16 sums per wave");
   const TI *rowptr = static_cast<const TI *>(a.rowptr);
   TY *y = static_cast<TY *>(a.y);
-  const int tid = threadIdx.x;
   for (int64_t i = 0; i < nr; i++) {
     int s, e;
     if (rows_lds) {
@@ -676,36 +823,7 @@ __device__ __forceinline__ void reduce_rows_mrhs_wg(
       s = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r0 + i) - e0);
       e = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r0 + i + 1) - e0);
     }
-    TA acc[KP];
-#pragma unroll
-    for (int jj = 0; jj < KP; jj++) acc[jj] = 0;
-    int q = s + tid;
-    for (; q + WG < e; q += 2 * WG) {
-      TA x0[KP], x1[KP];
-      load_xrow<KP>(xrc + static_cast<size_t>(c_s[q]) * KP, x0);
-      load_xrow<KP>(xrc + static_cast<size_t>(c_s[q + WG]) * KP, x1);
-      const TA v0 = v_s[q], v1 = v_s[q + WG];
-#pragma unroll
-      for (int jj = 0; jj < KP; jj++) {
-        acc[jj] += v0 * x0[jj];
-        acc[jj] += v1 * x1[jj];
-      }
-    }
-    if (q < e) {
-      TA x0[KP];
-      load_xrow<KP>(xrc + static_cast<size_t>(c_s[q]) * KP, x0);
-      const TA v0 = v_s[q];
-#pragma unroll
-      for (int jj = 0; jj < KP; jj++) acc[jj] += v0 * x0[jj];
-    }
-#pragma unroll
-    for (int jj = 0; jj < KP; jj++) acc[jj] = group_sum<64>(acc[jj]);
-    __syncthreads();
-    if ((tid & 63) == 0) {
-#pragma unroll
-      for (int jj = 0; jj < KP; jj++) red[jj * NW + (tid >> 6)] = acc[jj];
-    }
-    __syncthreads();
+    wg_row_sums<WG, KP, kPlain, TA, TX, TA, TI, true>(a, v_s, c_s, s, e, xrc, red, tid);
     if (tid < kk) {
       TA sum = 0;
 #pragma unroll
@@ -723,15 +841,36 @@ template <int WG, int KP, typename TX, typename TY, typename TA, typename TI>
 __device__ __forceinline__ void reduce_rows_mrhs_any(
     int n_s, const TA *v_s, const uint32_t *c_s, const int *rp_s,
     bool rows_lds, const FwdArgs &a, int64_t r0, int64_t e0, int64_t nr,
-    const TX *xrc, int64_t j0, int kk, TA alpha, TA beta, TA *red) {
+    const TX *xrc, int64_t j0, int kk, TA alpha, TA beta, TA *red, int tid) {
   if (nr <= 0) return;
   const int64_t avg = (n_s / nr) >> a.gshift;
   if (nr < WG / 64 && n_s / nr > 96) {
     reduce_rows_mrhs_wg<WG, KP, TX, TY, TA, TI>(v_s, c_s, rp_s, rows_lds, a, r0, e0, nr, xrc, j0,
-                                               kk, alpha, beta, red);
+                                               kk, alpha, beta, red, tid);
     return;
   }
-#define DRX_RM(G) reduce_rows_mrhs<WG, G, KP, TX, TY, TA, TI>(v_s, c_s, rp_s, rows_lds, a, r0, e0, nr, xrc, j0, kk, alpha, beta)
+  constexpr int KL = 16 / static_cast<int>(sizeof(TX));
+  if constexpr ((sizeof(TX) == 4 || sizeof(TX) == 8) && KP >= 2 * KL) {
+    // 16-byte pieces of the x rows, S lanes per entry (G entry lanes per
+    // row, G * S <= 64)
+    constexpr int GMAX = 64 / (KP / KL);
+    constexpr int G2 = 2 < GMAX ? 2 : GMAX, G4 = 4 < GMAX ? 4 : GMAX;
+    constexpr int G8 = 8 < GMAX ? 8 : GMAX, G16 = 16 < GMAX ? 16 : GMAX;
+#define DRX_RS(G) reduce_rows_mrhs_split<WG, G, KP, TX, TY, TA, TI>(v_s, c_s, rp_s, rows_lds, a, r0, e0, nr, xrc, j0, kk, alpha, beta, tid)
+    if (avg <= 3)
+      DRX_RS(1);
+    else if (avg <= 6)
+      DRX_RS(G2);
+    else if (avg <= 12)
+      DRX_RS(G4);
+    else if (avg <= 24)
+      DRX_RS(G8);
+    else
+      DRX_RS(G16);
+#undef DRX_RS
+    return;
+  }
+#define DRX_RM(G) reduce_rows_mrhs<WG, G, KP, TX, TY, TA, TI>(v_s, c_s, rp_s, rows_lds, a, r0, e0, nr, xrc, j0, kk, alpha, beta, tid)
   if (avg <= 3)
     DRX_RM(1);
   else if (avg <= 6)
@@ -760,7 +899,11 @@ __device__ __forceinline__ void process_block_mrhs(const FwdArgs &a,
   constexpr int T = CAP / WG;
   constexpr int ROWCAP = CAP / 2;
   constexpr int NW = WG / 64;
-  const int tid = threadIdx.x;
+  // The lane index as a value of this block: the batch kernel walks blocks
+  // in a loop, and values derived from the lane index for every lane-group
+  // size were hoisted out of it and held in registers (4 waves instead of 6).
+  int tid;
+  asm volatile("" : "=v"(tid) : "0"(static_cast<int>(threadIdx.x)));
   const TI *rowptr = static_cast<const TI *>(a.rowptr);
   const TI *colind = static_cast<const TI *>(a.colind);
   const TV *vals = static_cast<const TV *>(a.vals);
@@ -820,7 +963,7 @@ __device__ __forceinline__ void process_block_mrhs(const FwdArgs &a,
     const int kk = static_cast<int>(a.k - jc * KP < KP ? a.k - jc * KP : KP);
     reduce_rows_mrhs_any<WG, KP, TX, TY, TA, TI>(
         n_s, sm.v, sm.c, sm.rp_s, rows_lds, a, b.r0, b.e0, nr,
-        xr + jc * a.ncols * KP, jc * KP, kk, alpha, beta, sm.red);
+        xr + jc * a.ncols * KP, jc * KP, kk, alpha, beta, sm.red, tid);
   }
 
   if (b.es < e1) {
@@ -829,24 +972,8 @@ __device__ __forceinline__ void process_block_mrhs(const FwdArgs &a,
     for (int64_t jc = 0; jc < nchunk; jc++) {
       const int kk = static_cast<int>(a.k - jc * KP < KP ? a.k - jc * KP : KP);
       const TX *xrc = xr + jc * a.ncols * KP;
-      TA acc[KP];
-#pragma unroll
-      for (int jj = 0; jj < KP; jj++) acc[jj] = 0;
-      for (int64_t q = b.es + tid; q < e1; q += WG) {
-        const TA w = to_acc<TA>(long_val<MODE>(vals, perm, q));
-        TA xv[KP];
-        load_xrow<KP>(xrc + static_cast<size_t>(ld_stream(colind + q)) * KP, xv);
-#pragma unroll
-        for (int jj = 0; jj < KP; jj++) acc[jj] += w * xv[jj];
-      }
-#pragma unroll
-      for (int jj = 0; jj < KP; jj++) acc[jj] = group_sum<64>(acc[jj]);
-      __syncthreads();
-      if ((tid & 63) == 0) {
-#pragma unroll
-        for (int jj = 0; jj < KP; jj++) sm.red[jj * NW + (tid >> 6)] = acc[jj];
-      }
-      __syncthreads();
+      wg_row_sums<WG, KP, MODE, TV, TX, TA, TI, false>(a, nullptr, nullptr, b.es, e1, xrc,
+                                                      sm.red, tid);
       if (tid < kk) {
         TA s = 0;
 #pragma unroll
@@ -861,9 +988,22 @@ __device__ __forceinline__ void process_block_mrhs(const FwdArgs &a,
   __syncthreads();  // the next block overwrites the LDS window
 }
 
+// Waves per SIMD of the multi-RHS kernels: as many workgroups as the LDS
+// holds, at most the most that compile without spilling: 6 for sums of up
+// to 16 bytes per lane (73-78 VGPRs), 5 for wider ones taken in 16-byte
+// pieces (95), 4 for 16-bit x in wide chunks (whole chunks per lane).
+template <int WG, int CAP, int KP, typename TX, typename TA> constexpr int mrhs_waves() {
+  constexpr int lds = static_cast<int>(sizeof(MrhsSmem<WG, CAP, TA>));
+  constexpr int w = (160 * 1024) / lds * (WG / 256);
+  constexpr int most = KP * sizeof(TA) <= 16 ? 6 : (sizeof(TX) == 2 ? 4 : 5);
+  return w > most ? most : (w < 1 ? 1 : w);
+}
+
 template <int WG, int CAP, int KP, int MODE, typename TV, typename TX,
           typename TY, typename TA, typename TI>
-__global__ __launch_bounds__(WG) void k_spmv_mrhs(FwdArgs a) {
+__global__ __launch_bounds__(WG)
+__attribute__((amdgpu_waves_per_eu(mrhs_waves<WG, CAP, KP, TX, TA>(), 8)))
+void k_spmv_mrhs(FwdArgs a) {
   __shared__ MrhsSmem<WG, CAP, TA> sm;
   for (int64_t wi = blockIdx.x; wi < a.per_xcd * kXcds; wi += a.grid) {
     const int64_t blk = map_block(wi, a.per_xcd);
