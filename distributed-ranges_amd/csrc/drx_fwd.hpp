@@ -46,6 +46,7 @@ struct FwdArgs {
   double alpha, beta;
   int gshift;  // lane-group choice uses (mean row length >> gshift)
   int xstage;  // kPlain, k = 1: stage each block's x window in LDS
+  int rowgather;  // kPlain, k = 1: row-ordered gathers (kernels built with RG)
   const void *xr;  // k >= 2: x in row-major chunks of KP columns
   int64_t ncols;   // columns of the share (rows of one xr chunk)
 };
@@ -147,6 +148,55 @@ __device__ __forceinline__ void reduce_rows_any(int n_s, const TA *prod,
 #undef DRX_RR
 }
 
+// Row-ordered gathers (plain CSR, one right-hand side): one lane per row,
+// the block's rows in consecutive lanes. A lane reads its row's values and
+// columns from LDS (slots [rp_s[i], rp_s[i + 1]) of v_s / c_s), U at a time,
+// issues their U x gathers together and adds the products in entry order.
+// Where the rows are short and neighbouring rows have neighbouring columns
+// (stencils), one gather instruction of a wave reads x of 64 consecutive
+// rows' u-th neighbours: a few cache lines, where entry-ordered gathers of
+// the same entries span every neighbour's lines. Slots past a row's end
+// repeat its first entry (in range, and its x is gathered anyway) and add
+// nothing.
+template <int WG, int U, typename TX, typename TY, typename TA, typename TI>
+__device__ __forceinline__ void row_gather_sums(const TA *v_s, const uint32_t *c_s,
+                                                const int *rp_s, bool rows_lds,
+                                                const FwdArgs &a, int64_t r0,
+                                                int64_t e0, int64_t nr, TY *y,
+                                                TA alpha, TA beta) {
+  const TI *rowptr = static_cast<const TI *>(a.rowptr);
+  const rsrc_t rx = make_rsrc(a.x, a.xbytes);
+  for (int64_t i = threadIdx.x; i < nr; i += WG) {
+    int s, e;
+    if (rows_lds) {
+      s = rp_s[i];
+      e = rp_s[i + 1];
+    } else {
+      s = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r0 + i) - e0);
+      e = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r0 + i + 1) - e0);
+    }
+    TA acc = 0;
+    for (int q = s; q < e; q += U) {
+      uint32_t cc[U];
+      TA vv[U], xx[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int qq = q + u < e ? q + u : q;
+        cc[u] = c_s[qq];
+        vv[u] = v_s[qq];
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) xx[u] = to_acc<TA>(buf_load<TX>(rx, cc[u] * sizeof(TX)));
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (q + u < e) acc += vv[u] * xx[u];
+    }
+    TA out = alpha * acc;
+    if (beta != TA(0)) out += beta * to_acc<TA>(y[r0 + i]);
+    y[r0 + i] = from_acc<TY>(out);
+  }
+}
+
 template <int MODE, typename TV, typename TI>
 __device__ __forceinline__ TV long_val(const TV *vals, const TI *perm,
                                        int64_t q) {
@@ -157,11 +207,15 @@ __device__ __forceinline__ TV long_val(const TV *vals, const TI *perm,
 }
 
 // LDS of one workgroup of the forward kernels (row-map entries have the
-// index type).
-template <int WG, int CAP, bool ROWMAP, typename TA, typename TI> struct FwdSmem {
-  static constexpr int ROWCAP = CAP / 2;
+// index type; RG: the window's column indices for row-ordered gathers).
+template <int WG, int CAP, bool ROWMAP, typename TA, typename TI, bool RG = false>
+struct FwdSmem {
+  // rows of a block held in LDS (blocks with more read rowptr); row-ordered
+  // gathers serve shares of at least four entries per row on average
+  static constexpr int ROWCAP = RG ? CAP / 4 : CAP / 2;
   static constexpr int MAPCAP = ROWMAP ? ROWCAP : 1;
   TA prod[CAP];
+  uint32_t cs[RG ? CAP : 0];  // empty unless RG (the other kernels keep their LDS)
   TA ypre[MAPCAP];        // kRowMap, k = 1: the current y of the block's rows
   TA red[WG / 64];
   TI map_s[MAPCAP];       // kRowMap: output rows of the block
@@ -191,16 +245,18 @@ template <typename TV, typename U> __device__ __forceinline__ TV bits_to(U u) {
 // past the block (and past the arrays' last entry, inside that same aligned
 // group); such entries are read but never used.
 template <int WG, int CAP, int MODE, bool BUFX, typename TV, typename TX,
-          typename TY, typename TA, typename TI, int VW = 1, bool XS = false>
+          typename TY, typename TA, typename TI, int VW = 1, bool XS = false,
+          bool RG = false>
 __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
-                                              FwdSmem<WG, CAP, MODE == kRowMap, TA, TI> &sm) {
+                                              FwdSmem<WG, CAP, MODE == kRowMap, TA, TI, RG> &sm) {
   static_assert(VW == 1 || ((VW == 2 || VW == 4) && MODE != kPerm && sizeof(TI) == 4 &&
                             VW * sizeof(TV) <= 16 && VW <= kWinSlack),
                 "groups of entries: plain or row-map mode, 32-bit indices, at most 16 bytes "
                 "of values per group");
   static_assert(!XS || (MODE == kPlain && BUFX), "x staging: plain mode, buffer gathers");
+  static_assert(!RG || (MODE == kPlain && BUFX && !XS), "row gathers: plain mode, buffer gathers");
   constexpr int T = CAP / WG;
-  constexpr int ROWCAP = CAP / 2;
+  constexpr int ROWCAP = FwdSmem<WG, CAP, MODE == kRowMap, TA, TI, RG>::ROWCAP;
   TA *prod = sm.prod;
   int *rp_s = sm.rp_s;
   TI *map_s = sm.map_s;
@@ -367,6 +423,23 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
     // entry reads its x from there -- or from memory past xn. The products
     // equal the gathered ones bit for bit.
     bool staged = false;
+    if constexpr (RG) {
+      // Row-ordered gathers (k = 1; the host picks these kernels for shares
+      // of short rows): the window's values and columns go to LDS instead
+      // of products.
+#pragma unroll
+      for (int u = 0; u < T; u++) {
+        prod[slot(u)] = v[u];
+        sm.cs[slot(u)] = c[u];
+      }
+      __syncthreads();
+      // eight entries of a row per round (rows of 7 in one; measured against
+      // two rows of four per lane and one row of four: 3-D Laplacian 256^3
+      // BF16 174 vs 184 / 188 us, FP32 212 vs 219 / 221 us)
+      row_gather_sums<WG, 8, TX, TY, TA, TI>(prod + sh, sm.cs + sh, rp_s, rows_lds, a, b.r0,
+                                             b.e0, nr, y, alpha, beta);
+      __syncthreads();
+    }
     if constexpr (XS) {
       if (a.k == 1) {
         constexpr uint32_t XCAP = CAP * sizeof(TA) / sizeof(TX);
@@ -407,7 +480,7 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
       }
     }
 
-    for (int64_t j = 0; j < a.k; j++) {
+    for (int64_t j = 0; j < (RG ? 0 : a.k); j++) {
       const TX *xj = x + j * a.ldx;
       if (staged) {
       } else if constexpr (BUFX) {
@@ -508,22 +581,24 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
 // 8. Measured on MI355X: 7-8 waves raise the structured forms by 7-17 %
 // over the compiler's default (4-5 waves); trading LDS for an eighth FP64
 // workgroup spilled registers and lost 30 %.
-template <int WG, int CAP, int MODE, typename TA, typename TI> constexpr int fwd_waves() {
-  constexpr int lds = static_cast<int>(sizeof(FwdSmem<WG, CAP, MODE == kRowMap, TA, TI>));
+template <int WG, int CAP, int MODE, typename TA, typename TI, bool RG = false>
+constexpr int fwd_waves() {
+  constexpr int lds = static_cast<int>(sizeof(FwdSmem<WG, CAP, MODE == kRowMap, TA, TI, RG>));
   constexpr int wgs = (160 * 1024) / lds;
   constexpr int w = wgs * (WG / 256);
   return w > 8 ? 8 : (w < 1 ? 1 : w);
 }
 
-template <int WG, int CAP, int MODE, typename TV, typename TA, typename TI>
+template <int WG, int CAP, int MODE, typename TV, typename TA, typename TI, bool RG = false>
 constexpr int fwd_kernel_waves() {
-  constexpr int w = fwd_waves<WG, CAP, MODE, TA, TI>() -
+  constexpr int w = fwd_waves<WG, CAP, MODE, TA, TI, RG>() -
                     ((sizeof(TV) == 2 && MODE == kPlain) || MODE == kPerm ? 1 : 0);
   return sizeof(TA) == 8 && CAP / WG >= 8 && w > 6 ? 6 : w;
 }
 
 template <int WG, int CAP, int MODE, bool BUFX, typename TV, typename TX,
-          typename TY, typename TA, typename TI, int VW = 1, bool XS = false>
+          typename TY, typename TA, typename TI, int VW = 1, bool XS = false,
+          bool RG = false>
 __global__ __launch_bounds__(WG)
 // (16-bit values in plain mode take one wave fewer: at the LDS-bound count
 // their pair unpacking spilled registers; 3-D Laplacian BF16 390 -> 371 us.
@@ -532,13 +607,13 @@ __global__ __launch_bounds__(WG)
 // loads spilled at 8; band +-16 FP32 adjoint 563 -> 471 us.)
 // (FP64 sums with 8 or more entries per lane run at most 6 waves: their
 // gathers in halves of four need the registers.)
-__attribute__((amdgpu_waves_per_eu(fwd_kernel_waves<WG, CAP, MODE, TV, TA, TI>(), 8)))
+__attribute__((amdgpu_waves_per_eu(fwd_kernel_waves<WG, CAP, MODE, TV, TA, TI, RG>(), 8)))
 void k_spmv_fwd(FwdArgs a) {
-  __shared__ FwdSmem<WG, CAP, MODE == kRowMap, TA, TI> sm;
+  __shared__ FwdSmem<WG, CAP, MODE == kRowMap, TA, TI, RG> sm;
   for (int64_t wi = blockIdx.x; wi < a.per_xcd * kXcds; wi += a.grid) {
     const int64_t blk = map_block(wi, a.per_xcd);
     if (blk >= a.nblocks) continue;
-    process_block<WG, CAP, MODE, BUFX, TV, TX, TY, TA, TI, VW, XS>(a, blk, sm);
+    process_block<WG, CAP, MODE, BUFX, TV, TX, TY, TA, TI, VW, XS, RG>(a, blk, sm);
   }
 }
 

@@ -42,6 +42,8 @@ static std::atomic<int> g_fill{1};
 int tuned_fill() { return g_fill.load(); }
 static std::atomic<int> g_fwd_cap{0};
 int tuned_fwd_cap() { return g_fwd_cap.load(); }
+static std::atomic<int> g_rowgather{1};
+int tuned_rowgather() { return g_rowgather.load(); }
 
 int64_t tuned_block_nnz() {
   int64_t v = g_block_nnz.load();
@@ -402,6 +404,9 @@ int drx_tune_set(const char *key, int64_t value) {
       DRX_REQUIRE(value == 0 || value == 2048 || value == 4096,
                   "fwd_cap must be 0, 2048 or 4096");
       g_fwd_cap.store(static_cast<int>(value));
+    } else if (std::strcmp(key, "rowgather") == 0) {
+      DRX_REQUIRE(value >= 0 && value <= 2, "rowgather must be 0, 1 or 2");
+      g_rowgather.store(static_cast<int>(value));
     } else if (std::strcmp(key, "auto_panels") == 0) {
       DRX_REQUIRE(value == 0 || value == 1, "auto_panels must be 0 or 1");
       g_auto_panels.store(static_cast<int>(value));
@@ -441,6 +446,8 @@ int drx_tune_get(const char *key, int64_t *value) {
       *value = g_fill.load();
     else if (std::strcmp(key, "fwd_cap") == 0)
       *value = g_fwd_cap.load();
+    else if (std::strcmp(key, "rowgather") == 0)
+      *value = g_rowgather.load();
     else
       throw Error(DRX_ERR_INVALID, std::string("unknown tuning key: ") + key);
   });

@@ -98,6 +98,9 @@ int tuned_fill();
 // Forward window for plain CSR: 0 by value type, else 2048 (the plan's own
 // records) or 4096 (the wide records, where the plan holds them).
 int tuned_fwd_cap();
+// Row-ordered x gathers for plain CSR, k = 1: 0 never, 1 by the plan's row
+// lengths and column locality, 2 always.
+int tuned_rowgather();
 
 // Stream-ordered scratch from the library's per-device pool.
 void *scratch_alloc(size_t bytes, hipStream_t s);

@@ -69,6 +69,28 @@ static void launch_fwd_vw(int64_t cap, const FwdArgs &a, hipStream_t s) {
                        grid, dim3(1024), 0, s, a);
 }
 
+// Row-ordered gathers (plain CSR, k = 1) for the 2048- and 4096-entry
+// windows.
+template <bool BUFX, typename TV, typename TX, typename TY, typename TA, typename TI, int VW>
+static bool launch_fwd_rg(int64_t cap, const FwdArgs &a, hipStream_t s) {
+  const dim3 grid(static_cast<unsigned>(a.grid));
+  if (cap == 2048) {
+    hipLaunchKernelGGL((k_spmv_fwd<256, 2048, kPlain, BUFX, TV, TX, TY, TA, TI, VW, false, true>),
+                       grid, dim3(256), 0, s, a);
+    return true;
+  }
+  if (cap == 4096) {
+    if constexpr (wide_window<kPlain, TV, TA, TI>())
+      hipLaunchKernelGGL((k_spmv_fwd<256, 4096, kPlain, BUFX, TV, TX, TY, TA, TI, VW, false, true>),
+                         grid, dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_spmv_fwd<512, 4096, kPlain, BUFX, TV, TX, TY, TA, TI, VW, false, true>),
+                         grid, dim3(512), 0, s, a);
+    return true;
+  }
+  return false;
+}
+
 // Groups of VW consecutive entries per lane (32-bit indices, VW values in at
 // most 16 bytes, vals / colind aligned to a group; not the permuted-values
 // mode). Knob "vec": 0 (default) by type: groups of four for values of at
@@ -106,8 +128,10 @@ static void launch_fwd(const Plan &p, FwdArgs a, hipStream_t s) {
     // 4096: 263 us; FP32 band 4096: 217 us, 2048: 244 us.
     // 16-bit values take 4096 with groups of four (band +-16 BF16: 189 vs
     // 195 us; 3-D Laplacian equal), 2048 with single entries (4096: 296 us).
+    // Row-ordered gathers keep the plan's 2048-entry windows (eight waves;
+    // 4096: 3-D Laplacian FP32 222 vs 211 us, BF16 186 vs 173 us).
     int64_t want = tuned_fwd_cap();
-    if (want == 0 && wide_window<MODE, TV, TA, TI>() && tuned_wide() != 0 &&
+    if (want == 0 && !a.rowgather && wide_window<MODE, TV, TA, TI>() && tuned_wide() != 0 &&
         (sizeof(TV) == 4 || vw == 4))
       want = 4096;
     const BlockRec *rec = nullptr;
@@ -123,6 +147,14 @@ static void launch_fwd(const Plan &p, FwdArgs a, hipStream_t s) {
       a.per_xcd = g.per_xcd;
       a.grid = g.grid;
       cap = want;
+    }
+  }
+  if constexpr (MODE == kPlain && BUFX) {
+    if (a.rowgather && a.k == 1) {
+      if constexpr (sizeof(TI) == 4 && 4 * sizeof(TV) <= 16) {
+        if (vw == 4 && launch_fwd_rg<BUFX, TV, TX, TY, TA, TI, 4>(cap, a, s)) return;
+      }
+      if (vw == 1 && launch_fwd_rg<BUFX, TV, TX, TY, TA, TI, 1>(cap, a, s)) return;
     }
   }
   // x staged in LDS (knob xstage 2): plain CSR with x of the values' type
@@ -246,6 +278,18 @@ static void run_spmv_impl(const Plan &p, int vtype, const void *colind,
   // gathers beat it on bands (+-16 FP32: 241 vs 286 us per call).
   if constexpr (MODE == kPlain) {
     if (k == 1 && bufx && p.nnz > 0) a.xstage = tuned_xstage() == 2;
+    // Row-ordered gathers for shares of short rows (longest at most 16
+    // entries, at least 4 on average; knob rowgather), unless x staging is
+    // forced. Measured (us per call, against entry-ordered gathers): 3-D
+    // Laplacian 256^3 FP64 391 -> 381, FP32 223 -> 212, BF16 201 -> 173;
+    // 2-D Laplacian 1000^2 FP64 21.5 -> 21.1; random columns, 4..12 entries
+    // per row, 16M rows, FP32 2362 -> 2350; band +-16 (33 per row, not
+    // picked) FP32 220 -> 226.
+    if (k == 1 && bufx && p.nnz > 0) {
+      const int rg = tuned_rowgather();
+      a.rowgather = !a.xstage && (rg == 2 || (rg == 1 && p.lmax > 0 && p.lmax <= 16 &&
+                                              p.nnz >= 4 * p.nrows));
+    }
   }
   if (vtype == DRX_F64 && xtype == DRX_F64 && ytype == DRX_F64)
     launch_fwd_t<MODE, double, double, double, double>(p, a, bufx, s);

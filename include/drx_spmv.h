@@ -110,8 +110,18 @@ const char *drx_last_error(void);
  *   "fwd_cap"     forward window of plain CSR with one right-hand side: 0
  *                 (default) 4096 for FP32 values, and for 16-bit values
  *                 streamed in groups of four, with 32-bit indices (see
- *                 "wide"), else the plan's own (2 block_nnz); 2048 or 4096
- *                 force one where the plan holds those records.
+ *                 "wide") unless "rowgather" applies, else the plan's own
+ *                 (2 block_nnz); 2048 or 4096 force one where the plan
+ *                 holds those records.
+ *   "rowgather"   row-ordered x gathers for plain CSR with one right-hand
+ *                 side: each block's values and columns are staged in LDS
+ *                 and each lane sums one row, gathering its x entries in
+ *                 row order (stencils: a wave's gather reads 64 consecutive
+ *                 rows' neighbours). 1 (default) for plans whose longest
+ *                 row has at most 16 entries and whose rows average at
+ *                 least 4 (2048-entry windows); 2 always; 0 never. Sums
+ *                 in entry order, so results may differ from 0 in the last
+ *                 bits (both within the oracle tolerance).
  *   "gemm_tile"   drx_gemm workgroup tile: 1 128 x 128; 2 256 x 128; 3 and
  *                 4 the same with stages twice as deep in k; 5 256 x 256
  *                 (FP32; FP64 takes 4); 6 128 x 128 loading stages straight

@@ -494,7 +494,8 @@ def test_batch_matches_single_calls_and_oracle(orc, k):
     b.run()
     torch.cuda.synchronize()
     for A, x, y, d, dx in zip(mats, xs, ys, dys, dxs):
-        with drx.tuned(wide=0):  # single calls on the batch's 1024-entry blocks
+        # single calls on the batch's 1024-entry blocks and entry-ordered gathers
+        with drx.tuned(wide=0, rowgather=0):
             assert torch.equal(y, A.matvec(x))
             assert torch.equal(dx, A.rmatvec(d))
         rp, ci, v = A.rowptr.cpu().numpy(), A.colind.cpu().numpy(), A.vals.cpu().numpy()
@@ -660,7 +661,7 @@ def test_x_staging_bitexact(orc, vdt, itype):
         x = drx.gen.uniform(A.shape[1], vdt, -1, 1, seed=11)
         ys = {}
         for xs in (0, 2, 1):
-            with drx.tuned(xstage=xs):
+            with drx.tuned(xstage=xs, rowgather=0):
                 ys[xs] = Ad.matvec(x)
         torch.cuda.synchronize()
         assert torch.equal(ys[0], ys[2]), name
@@ -760,3 +761,90 @@ def test_fp32_wide_blocks(orc, wide, lens_kind):
     expect = 1.5 * ys - 0.5 * y0.astype(np.float64)
     den = 1.5 * absy + 0.5 * np.abs(y0)
     assert (np.abs(y.double().cpu().numpy() - expect) <= 1e-5 * den + 1e-300).all()
+
+
+def _short_rows(m, n, seed, lmax, local, empty=0.15, extra=()):
+    """Rows of 1..lmax entries (a share `empty` of them empty), sorted
+    distinct columns: within 64 columns around the row's diagonal (local) or
+    anywhere; `extra` appends rows of the given lengths."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(1, lmax + 1, m)
+    lens[rng.random(m) < empty] = 0
+    lens = list(lens) + list(extra)
+    cols = []
+    for r, L in enumerate(lens):
+        if local and L <= 64:
+            lo = min(max(0, r * n // len(lens) - 32), n - 64)
+            cols.append(np.sort(rng.choice(64, L, replace=False)) + lo)
+        else:
+            cols.append(np.sort(rng.choice(n, L, replace=False)))
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    return rp, np.concatenate(cols).astype(np.int64), rng.uniform(-1, 1, int(rp[-1])), len(lens)
+
+
+def _row_gather_cases():
+    """(name, rowptr, colind, vals, shape, auto): `auto` says whether the
+    default picks row-ordered gathers (longest row <= 16, >= 4 per row)."""
+    out = []
+    L = drx.gen.laplacian(3, 22, vdtype=torch.float64)
+    out.append(("lap3d", L.rowptr.cpu().numpy(), L.colind.cpu().numpy(), L.vals.cpu().numpy(),
+                L.shape, True))
+    rp, ci, v, m = _short_rows(20000, 30000, 1, 16, True, empty=0.1)
+    out.append(("short_local", rp, ci, v, (m, 30000), True))
+    rp, ci, v, m = _short_rows(20000, 50000, 2, 12, False, empty=0.05)
+    out.append(("short_random", rp, ci, v, (m, 50000), True))
+    # rows longer than eight entries (several rounds per lane), longer than
+    # the window (the whole-workgroup path) and empty: forced only
+    rp, ci, v, m = _short_rows(6000, 30000, 3, 40, True, extra=(9000, 3, 0, 2500, 17))
+    out.append(("long_rows", rp, ci, v, (m, 30000), False))
+    # rows of 0..2 entries: more rows per block than the LDS holds (row
+    # offsets read from rowptr)
+    rp, ci, v, m = _short_rows(30000, 30000, 4, 2, True, empty=0.3)
+    out.append(("tiny_rows", rp, ci, v, (m, 30000), False))
+    return out
+
+
+@pytest.mark.parametrize("vdt", [torch.float64, torch.float32, torch.bfloat16, torch.float16])
+@pytest.mark.parametrize("itype", [torch.int32, torch.int64])
+def test_row_ordered_gathers(orc, vdt, itype):
+    """Row-ordered gathers (knob rowgather: values and columns of each block
+    staged in LDS, one lane per row) against the oracle, with alpha and beta:
+    stencils, short local and random rows, rows longer than eight entries
+    and than the window, empty rows, blocks with more rows than the LDS
+    holds; groups of four entries per lane and single entries (vals one
+    element into a larger buffer). The default picks them exactly for the
+    eligible shares (same bits as forcing them, or as turning them off)."""
+    low = vdt in (torch.bfloat16, torch.float16)
+    sdt = np.float64 if vdt == torch.float64 else np.float32
+    tol = 1e-12 if vdt == torch.float64 else 1e-5
+    for name, rp, ci, v, shape, auto in _row_gather_cases():
+        m, n = shape
+        Ad = upload(rp, ci, v.astype(sdt), shape, itype=itype)
+        vq = Ad.vals.to(vdt)
+        xdt = torch.float64 if vdt == torch.float64 else torch.float32
+        xq = drx.gen.uniform(n, xdt, -1, 1, seed=21).to(vdt)
+        ydt = torch.float32 if low else vdt
+        y0 = drx.gen.uniform(m, ydt, -1, 1, seed=22)
+        outs = {}
+        for rg, vals_off in ((2, 0), (2, 1), (1, 0), (0, 0)):
+            vv = vq
+            if vals_off:
+                buf = torch.zeros(vq.numel() + vals_off, dtype=vdt, device=DEV)
+                buf[vals_off:] = vq
+                vv = buf[vals_off:]
+            y = y0.clone()
+            with drx.tuned(rowgather=rg):
+                Ad.matvec(xq, y=y, vals=vv, alpha=1.5, beta=-0.5)
+            outs[(rg, vals_off)] = y
+        torch.cuda.synchronize()
+        assert torch.equal(outs[(1, 0)], outs[(2, 0)] if auto else outs[(0, 0)]), name
+        vd = vq.double().cpu().numpy()
+        xd = xq.double().cpu().numpy()
+        ys, absy = orc.spmv(rp, ci, vd, xd)
+        y0d = y0.double().cpu().numpy()
+        expect = 1.5 * ys - 0.5 * y0d
+        den = 1.5 * absy + 0.5 * np.abs(y0d)
+        for key, y in outs.items():
+            got = y.double().cpu().numpy()
+            assert (np.abs(got - expect) <= tol * den + 1e-300).all(), (name, key)
+        Ad.close()
