@@ -158,8 +158,9 @@ __device__ __forceinline__ void reduce_rows_any(int n_s, const TA *prod,
 // the same entries span every neighbour's lines. Slots past a row's end
 // repeat its first entry (in range, and its x is gathered anyway) and add
 // nothing.
-template <int WG, int U, typename TX, typename TY, typename TA, typename TI>
-__device__ __forceinline__ void row_gather_sums(const TA *v_s, const uint32_t *c_s,
+template <int WG, int U, typename TX, typename TY, typename TA, typename TI,
+          typename VS = TA, typename CS = uint32_t>
+__device__ __forceinline__ void row_gather_sums(const VS *v_s, const CS *c_s,
                                                 const int *rp_s, bool rows_lds,
                                                 const FwdArgs &a, int64_t r0,
                                                 int64_t e0, int64_t nr, TY *y,
@@ -182,8 +183,8 @@ __device__ __forceinline__ void row_gather_sums(const TA *v_s, const uint32_t *c
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int qq = q + u < e ? q + u : q;
-        cc[u] = c_s[qq];
-        vv[u] = v_s[qq];
+        cc[u] = col32(c_s[qq]);
+        vv[u] = to_acc<TA>(v_s[qq]);
       }
 #pragma unroll
       for (int u = 0; u < U; u++) xx[u] = to_acc<TA>(buf_load<TX>(rx, cc[u] * sizeof(TX)));
@@ -614,6 +615,162 @@ void k_spmv_fwd(FwdArgs a) {
     const int64_t blk = map_block(wi, a.per_xcd);
     if (blk >= a.nblocks) continue;
     process_block<WG, CAP, MODE, BUFX, TV, TX, TY, TA, TI, VW, XS, RG>(a, blk, sm);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row-ordered gathers with each block's window copied straight into LDS
+// (plain CSR, one right-hand side): buffer loads with the LDS flag write a
+// wave's 64 lanes x D bytes to a contiguous 64 D-byte span of LDS, so the
+// window's values and column indices land in LDS exactly as they lie in
+// memory, with no register round trip and no LDS write instructions. The
+// rows are then summed as in row_gather_sums, reading the stored types.
+// ---------------------------------------------------------------------------
+template <int WG, int CAP, typename TV, typename TI, typename TA> struct RowsSmem {
+  static constexpr int ROWCAP = CAP / 4;
+  alignas(16) TV v[CAP];
+  alignas(16) TI c[CAP];
+  int rp_s[ROWCAP + 1];
+  TA red[WG / 64];
+};
+
+// Bytes per lane of the LDS copy of an array of E: 16, or 4 for 2-byte
+// elements (a 16-byte unit aligned to 8 bytes could cross a page past the
+// array's end).
+template <typename E> constexpr int lds_copy_width() { return sizeof(E) >= 4 ? 16 : 4; }
+
+// Copies the block's elements [e0, e0 + n) of `src` into `dst` from the
+// copy unit at or below e0 (returns the shift e0 - that unit's first
+// element, below kWinSlack); whole units, so the last one may extend past
+// the block and past the array's last element inside that aligned unit
+// (the caller guarantees `src` aligned to a unit). Rounds past the
+// block's units are skipped; lanes past them read nothing (range check).
+template <int WG, int CAP, typename E>
+__device__ __forceinline__ int lds_window(const E *src, int64_t e0, int n, E *dst, int tid) {
+  constexpr int D = lds_copy_width<E>();
+  constexpr int A = D / static_cast<int>(sizeof(E));
+  constexpr int RB = WG * D;
+  constexpr int R = CAP * static_cast<int>(sizeof(E)) / RB;
+  static_assert(R * RB == CAP * static_cast<int>(sizeof(E)), "whole rounds per window");
+  static_assert(A <= kWinSlack, "the shifted window fits CAP entries");
+  const int sh = static_cast<int>(e0 & (A - 1));
+  const uint32_t need = static_cast<uint32_t>((sh + n) * sizeof(E) + D - 1) / D * D;
+  const rsrc_t r = make_rsrc(src + (e0 - sh), need);
+  char *base = reinterpret_cast<char *>(dst) + (tid >> 6) * 64 * D;
+#pragma unroll
+  for (int g = 0; g < R; g++) {
+    if (static_cast<uint32_t>(g * RB) < need) {
+      auto *l = (__attribute__((address_space(3))) void *)(base + g * RB);
+      const uint32_t off = static_cast<uint32_t>((g * WG + tid) * D);
+      if constexpr (D == 16)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(r, l, 16, off, 0, 0, kNT);
+      else
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(r, l, 4, off, 0, 0, kNT);
+    }
+  }
+  return sh;
+}
+
+template <int WG, int CAP, typename TV, typename TX, typename TY, typename TA, typename TI>
+__device__ __forceinline__ void process_block_rows(const FwdArgs &a, int64_t blk,
+                                                   RowsSmem<WG, CAP, TV, TI, TA> &sm, int tid) {
+  constexpr int ROWCAP = RowsSmem<WG, CAP, TV, TI, TA>::ROWCAP;
+  const TI *rowptr = static_cast<const TI *>(a.rowptr);
+  const TI *colind = static_cast<const TI *>(a.colind);
+  const TV *vals = static_cast<const TV *>(a.vals);
+  const TX *x = static_cast<const TX *>(a.x);
+  TY *y = static_cast<TY *>(a.y);
+  const TA alpha = static_cast<TA>(a.alpha), beta = static_cast<TA>(a.beta);
+  const BlockRec b = a.rec[blk];
+  const int64_t r1 = a.rec[blk + 1].r0, e1 = a.rec[blk + 1].e0;
+  if (b.r0 >= r1) return;
+  const int n_s = static_cast<int>(b.es - b.e0);
+  const int64_t nr = (b.es == e1 ? r1 : r1 - 1) - b.r0;
+  const bool rows_lds = nr < ROWCAP;
+  const int shv = lds_window<WG, CAP>(vals, b.e0, n_s, sm.v, tid);
+  const int shc = lds_window<WG, CAP>(colind, b.e0, n_s, sm.c, tid);
+  if (rows_lds) {
+    // Row offsets 0..nr: every lane's loads issued before the first wait
+    // (indices clamped to valid rows; row_begin without its branch).
+    constexpr int NI = (ROWCAP + WG) / WG;
+    TI raw[NI];
+#pragma unroll
+    for (int k = 0; k < NI; k++) {
+      if (k * WG > nr) break;
+      const int64_t i = tid + k * WG;
+      const int64_t r = b.r0 + (i <= nr ? i : nr);
+      raw[k] = rowptr[r < a.nrows || a.sent ? r : r - 1];
+    }
+#pragma unroll
+    for (int k = 0; k < NI; k++) {
+      if (k * WG > nr) break;
+      const int64_t i = tid + k * WG;
+      if (i <= nr) {
+        const int64_t r = b.r0 + i;
+        int64_t v0 = a.nnz;
+        if (r < a.nrows || a.sent) {
+          v0 = static_cast<int64_t>(raw[k]) - a.base;
+          v0 = v0 < 0 ? 0 : (v0 > a.nnz ? a.nnz : v0);
+        }
+        sm.rp_s[i] = static_cast<int>(v0 - b.e0);
+      }
+    }
+  }
+  // the LDS copies count as vector-memory loads
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  row_gather_sums<WG, 8, TX, TY, TA, TI, TV, TI>(sm.v + shv, sm.c + shc, sm.rp_s, rows_lds, a,
+                                                 b.r0, b.e0, nr, y, alpha, beta);
+  if (b.es < e1) {
+    // Long last row: the whole workgroup, strided coalesced loads.
+    const int64_t last = r1 - 1;
+    // (the sum order of process_block's long-row path: the same bits)
+    TA acc0 = 0, acc1 = 0;
+    int64_t q = b.es + tid;
+    for (; q + 3 * WG < e1; q += 4 * WG) {
+      const TV w0 = ld_stream(vals + q), w1 = ld_stream(vals + q + WG);
+      const TV w2 = ld_stream(vals + q + 2 * WG), w3 = ld_stream(vals + q + 3 * WG);
+      const TI c0 = ld_stream(colind + q), c1 = ld_stream(colind + q + WG);
+      const TI c2 = ld_stream(colind + q + 2 * WG), c3 = ld_stream(colind + q + 3 * WG);
+      const TA x0 = to_acc<TA>(x[static_cast<int64_t>(c0)]);
+      const TA x1 = to_acc<TA>(x[static_cast<int64_t>(c1)]);
+      const TA x2 = to_acc<TA>(x[static_cast<int64_t>(c2)]);
+      const TA x3 = to_acc<TA>(x[static_cast<int64_t>(c3)]);
+      acc0 += to_acc<TA>(w0) * x0;
+      acc1 += to_acc<TA>(w1) * x1;
+      acc0 += to_acc<TA>(w2) * x2;
+      acc1 += to_acc<TA>(w3) * x3;
+    }
+    for (; q < e1; q += WG)
+      acc0 += to_acc<TA>(ld_stream(vals + q)) *
+              to_acc<TA>(x[static_cast<int64_t>(ld_stream(colind + q))]);
+    const TA s = block_sum<WG>(acc0 + acc1, sm.red);
+    if (tid == 0) {
+      TA out = alpha * s;
+      if (beta != TA(0)) out += beta * to_acc<TA>(y[last]);
+      y[last] = from_acc<TY>(out);
+    }
+  }
+  __syncthreads();  // the next block overwrites the LDS window
+}
+
+template <int WG, int CAP, typename TV, typename TI, typename TA> constexpr int rows_waves() {
+  constexpr int lds = static_cast<int>(sizeof(RowsSmem<WG, CAP, TV, TI, TA>));
+  constexpr int w = (160 * 1024) / lds * (WG / 256);
+  return w > 8 ? 8 : (w < 1 ? 1 : w);
+}
+
+template <int WG, int CAP, typename TV, typename TX, typename TY, typename TA, typename TI>
+__global__ __launch_bounds__(WG)
+__attribute__((amdgpu_waves_per_eu(rows_waves<WG, CAP, TV, TI, TA>(), 8)))
+void k_spmv_rows(FwdArgs a) {
+  __shared__ RowsSmem<WG, CAP, TV, TI, TA> sm;
+  int tid;
+  asm volatile("" : "=v"(tid) : "0"(static_cast<int>(threadIdx.x)));
+  for (int64_t wi = blockIdx.x; wi < a.per_xcd * kXcds; wi += a.grid) {
+    const int64_t blk = map_block(wi, a.per_xcd);
+    if (blk >= a.nblocks) continue;
+    process_block_rows<WG, CAP, TV, TX, TY, TA, TI>(a, blk, sm, tid);
   }
 }
 

@@ -405,7 +405,7 @@ int drx_tune_set(const char *key, int64_t value) {
                   "fwd_cap must be 0, 2048 or 4096");
       g_fwd_cap.store(static_cast<int>(value));
     } else if (std::strcmp(key, "rowgather") == 0) {
-      DRX_REQUIRE(value >= 0 && value <= 2, "rowgather must be 0, 1 or 2");
+      DRX_REQUIRE(value >= 0 && value <= 3, "rowgather must be in [0, 3]");
       g_rowgather.store(static_cast<int>(value));
     } else if (std::strcmp(key, "auto_panels") == 0) {
       DRX_REQUIRE(value == 0 || value == 1, "auto_panels must be 0 or 1");

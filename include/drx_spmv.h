@@ -119,9 +119,13 @@ const char *drx_last_error(void);
  *                 row order (stencils: a wave's gather reads 64 consecutive
  *                 rows' neighbours). 1 (default) for plans whose longest
  *                 row has at most 16 entries and whose rows average at
- *                 least 4 (2048-entry windows); 2 always; 0 never. Sums
- *                 in entry order, so results may differ from 0 in the last
- *                 bits (both within the oracle tolerance).
+ *                 least 4 (2048-entry windows), copying the window straight
+ *                 into LDS where vals and colind are aligned to 16 bytes (4
+ *                 for 16-bit values), else through registers; 2 always,
+ *                 through registers; 3 always, straight copies where
+ *                 aligned; 0 never. Sums in entry order, so results may
+ *                 differ from 0 in the last bits (all within the oracle
+ *                 tolerance).
  *   "gemm_tile"   drx_gemm workgroup tile: 1 128 x 128; 2 256 x 128; 3 and
  *                 4 the same with stages twice as deep in k; 5 256 x 256
  *                 (FP32; FP64 takes 4); 6 128 x 128 loading stages straight

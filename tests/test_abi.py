@@ -97,7 +97,7 @@ def test_gemm_argument_validation_without_gpu():
     assert L.drx_tune_set(b"xstage", 3) == bad
     assert L.drx_tune_set(b"xstage", 2) == 0 and L.drx_tune_set(b"xstage", 1) == 0
     assert L.drx_tune_get(b"rowgather", ctypes.byref(v)) == 0 and v.value == 1
-    assert L.drx_tune_set(b"rowgather", 3) == bad
+    assert L.drx_tune_set(b"rowgather", 4) == bad
     assert L.drx_tune_set(b"rowgather", 2) == 0 and L.drx_tune_set(b"rowgather", 1) == 0
 
 

@@ -811,9 +811,12 @@ def test_row_ordered_gathers(orc, vdt, itype):
     staged in LDS, one lane per row) against the oracle, with alpha and beta:
     stencils, short local and random rows, rows longer than eight entries
     and than the window, empty rows, blocks with more rows than the LDS
-    holds; groups of four entries per lane and single entries (vals one
-    element into a larger buffer). The default picks them exactly for the
-    eligible shares (same bits as forcing them, or as turning them off)."""
+    holds. The window copied straight into LDS (3) and through registers
+    (2: groups of four entries per lane, or single entries when vals start
+    one element into a larger buffer) give the same bits. The default (1)
+    picks them exactly for the eligible shares: the straight copy for
+    aligned vals, the register path otherwise, entry-ordered gathers (0) for
+    the rest."""
     low = vdt in (torch.bfloat16, torch.float16)
     sdt = np.float64 if vdt == torch.float64 else np.float32
     tol = 1e-12 if vdt == torch.float64 else 1e-5
@@ -826,7 +829,7 @@ def test_row_ordered_gathers(orc, vdt, itype):
         ydt = torch.float32 if low else vdt
         y0 = drx.gen.uniform(m, ydt, -1, 1, seed=22)
         outs = {}
-        for rg, vals_off in ((2, 0), (2, 1), (1, 0), (0, 0)):
+        for rg, vals_off in ((3, 0), (2, 0), (2, 1), (1, 0), (1, 1), (0, 0), (0, 1)):
             vv = vq
             if vals_off:
                 buf = torch.zeros(vq.numel() + vals_off, dtype=vdt, device=DEV)
@@ -837,7 +840,9 @@ def test_row_ordered_gathers(orc, vdt, itype):
                 Ad.matvec(xq, y=y, vals=vv, alpha=1.5, beta=-0.5)
             outs[(rg, vals_off)] = y
         torch.cuda.synchronize()
-        assert torch.equal(outs[(1, 0)], outs[(2, 0)] if auto else outs[(0, 0)]), name
+        assert torch.equal(outs[(3, 0)], outs[(2, 0)]), name
+        assert torch.equal(outs[(1, 0)], outs[(3, 0)] if auto else outs[(0, 0)]), name
+        assert torch.equal(outs[(1, 1)], outs[(2, 1)] if auto else outs[(0, 1)]), name
         vd = vq.double().cpu().numpy()
         xd = xq.double().cpu().numpy()
         ys, absy = orc.spmv(rp, ci, vd, xd)
