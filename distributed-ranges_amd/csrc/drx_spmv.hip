@@ -69,6 +69,13 @@ static void launch_fwd_vw(int64_t cap, const FwdArgs &a, hipStream_t s) {
                        grid, dim3(1024), 0, s, a);
 }
 
+// The row-gather kernel that copies windows straight into LDS needs vals
+// aligned to its copy unit and colind to 16 bytes.
+template <typename TV> static bool rows_lds_copy(const FwdArgs &a) {
+  return reinterpret_cast<uintptr_t>(a.vals) % lds_copy_width<TV>() == 0 &&
+         reinterpret_cast<uintptr_t>(a.colind) % 16 == 0;
+}
+
 // Row-ordered gathers (plain CSR, k = 1) for the 2048- and 4096-entry
 // windows.
 template <bool BUFX, typename TV, typename TX, typename TY, typename TA, typename TI, int VW>
@@ -128,9 +135,15 @@ static void launch_fwd(const Plan &p, FwdArgs a, hipStream_t s) {
     // 4096: 263 us; FP32 band 4096: 217 us, 2048: 244 us.
     // 16-bit values take 4096 with groups of four (band +-16 BF16: 189 vs
     // 195 us; 3-D Laplacian equal), 2048 with single entries (4096: 296 us).
-    // Row-ordered gathers keep the plan's 2048-entry windows (eight waves;
-    // 4096: 3-D Laplacian FP32 222 vs 211 us, BF16 186 vs 173 us).
+    // Row-ordered gathers keep the plan's 2048-entry windows, except 16-bit
+    // values copied straight into LDS, which take 4096 (3-D Laplacian 256^3,
+    // us per call, windows of 1024 / 2048 / 4096 entries: FP64 361 / 351 /
+    // 362, FP32 218 / 199 / 210, BF16 208 / 173 / 164; 2-D Laplacian 1000^2
+    // FP64 20.5 / 19.8 / 22.8; through registers 4096 is slower for every
+    // type).
     int64_t want = tuned_fwd_cap();
+    if (want == 0 && a.rowgather == 1 && sizeof(TV) == 2 && rows_lds_copy<TV>(a))
+      want = 4096;
     if (want == 0 && !a.rowgather && wide_window<MODE, TV, TA, TI>() && tuned_wide() != 0 &&
         (sizeof(TV) == 4 || vw == 4))
       want = 4096;
@@ -156,11 +169,17 @@ static void launch_fwd(const Plan &p, FwdArgs a, hipStream_t s) {
       // Measured against the register path (us per call): 3-D Laplacian
       // 256^3 FP64 382 -> 354, FP32 208 -> 204, BF16 174 -> 172; 2-D
       // Laplacian 1000^2 FP64 21.2 -> 20.1.
-      if (a.rowgather != 2 && cap == 2048 &&
-          reinterpret_cast<uintptr_t>(a.vals) % lds_copy_width<TV>() == 0 &&
-          reinterpret_cast<uintptr_t>(a.colind) % 16 == 0) {
-        hipLaunchKernelGGL((k_spmv_rows<256, 2048, TV, TX, TY, TA, TI>), dim3(a.grid), dim3(256),
-                           0, s, a);
+      if (a.rowgather != 2 && (cap == 1024 || cap == 2048 || cap == 4096) &&
+          rows_lds_copy<TV>(a)) {
+        if (cap == 1024)
+          hipLaunchKernelGGL((k_spmv_rows<256, 1024, TV, TX, TY, TA, TI>), dim3(a.grid),
+                             dim3(256), 0, s, a);
+        else if (cap == 2048)
+          hipLaunchKernelGGL((k_spmv_rows<256, 2048, TV, TX, TY, TA, TI>), dim3(a.grid),
+                             dim3(256), 0, s, a);
+        else
+          hipLaunchKernelGGL((k_spmv_rows<256, 4096, TV, TX, TY, TA, TI>), dim3(a.grid),
+                             dim3(256), 0, s, a);
         return;
       }
       if constexpr (sizeof(TI) == 4 && 4 * sizeof(TV) <= 16) {
