@@ -198,6 +198,71 @@ __device__ __forceinline__ void row_gather_sums(const VS *v_s, const CS *c_s,
   }
 }
 
+// G lanes per row (rows of more than a few entries): lane j of a row's group
+// takes entries j, j + G, ..., U of them per round with their gathers in
+// flight, and the group's sums are added with DPP / shuffles.
+template <int WG, int G, int U, typename TX, typename TY, typename TA, typename TI,
+          typename VS, typename CS>
+__device__ __forceinline__ void row_group_sums(const VS *v_s, const CS *c_s, const int *rp_s,
+                                               bool rows_lds, const FwdArgs &a, int64_t r0,
+                                               int64_t e0, int64_t nr, TY *y, TA alpha,
+                                               TA beta) {
+  const TI *rowptr = static_cast<const TI *>(a.rowptr);
+  const rsrc_t rx = make_rsrc(a.x, a.xbytes);
+  const int lane = threadIdx.x & (G - 1);
+  for (int64_t i = threadIdx.x / G; i < nr; i += WG / G) {
+    int s, e;
+    if (rows_lds) {
+      s = rp_s[i];
+      e = rp_s[i + 1];
+    } else {
+      s = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r0 + i) - e0);
+      e = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r0 + i + 1) - e0);
+    }
+    TA acc = 0;
+    for (int q = s + lane; q < e; q += G * U) {
+      uint32_t cc[U];
+      TA vv[U], xx[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int qq = q + u * G < e ? q + u * G : q;
+        cc[u] = col32(c_s[qq]);
+        vv[u] = to_acc<TA>(v_s[qq]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) xx[u] = to_acc<TA>(buf_load<TX>(rx, cc[u] * sizeof(TX)));
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (q + u * G < e) acc += vv[u] * xx[u];
+    }
+    acc = group_sum<G>(acc);
+    if (lane == 0) {
+      TA out = alpha * acc;
+      if (beta != TA(0)) out += beta * to_acc<TA>(y[r0 + i]);
+      y[r0 + i] = from_acc<TY>(out);
+    }
+  }
+}
+
+// Row-ordered sums of a block by its mean row length: one lane per row up
+// to 12 entries, else groups of 8 or 16 lanes.
+template <int WG, typename TX, typename TY, typename TA, typename TI, typename VS, typename CS>
+__device__ __forceinline__ void row_sums_any(int n_s, const VS *v_s, const CS *c_s,
+                                             const int *rp_s, bool rows_lds, const FwdArgs &a,
+                                             int64_t r0, int64_t e0, int64_t nr, TY *y, TA alpha,
+                                             TA beta) {
+  const int64_t avg = nr > 0 ? n_s / nr : 0;
+  if (avg <= 12)
+    row_gather_sums<WG, 8, TX, TY, TA, TI, VS, CS>(v_s, c_s, rp_s, rows_lds, a, r0, e0, nr, y,
+                                                   alpha, beta);
+  else if (avg <= 40)
+    row_group_sums<WG, 8, 5, TX, TY, TA, TI, VS, CS>(v_s, c_s, rp_s, rows_lds, a, r0, e0, nr, y,
+                                                     alpha, beta);
+  else
+    row_group_sums<WG, 16, 5, TX, TY, TA, TI, VS, CS>(v_s, c_s, rp_s, rows_lds, a, r0, e0, nr, y,
+                                                      alpha, beta);
+}
+
 template <int MODE, typename TV, typename TI>
 __device__ __forceinline__ TV long_val(const TV *vals, const TI *perm,
                                        int64_t q) {
@@ -437,8 +502,8 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
       // eight entries of a row per round (rows of 7 in one; measured against
       // two rows of four per lane and one row of four: 3-D Laplacian 256^3
       // BF16 174 vs 184 / 188 us, FP32 212 vs 219 / 221 us)
-      row_gather_sums<WG, 8, TX, TY, TA, TI>(prod + sh, sm.cs + sh, rp_s, rows_lds, a, b.r0,
-                                             b.e0, nr, y, alpha, beta);
+      row_sums_any<WG, TX, TY, TA, TI>(n_s, prod + sh, sm.cs + sh, rp_s, rows_lds, a, b.r0, b.e0,
+                                       nr, y, alpha, beta);
       __syncthreads();
     }
     if constexpr (XS) {
@@ -654,7 +719,8 @@ __device__ __forceinline__ int lds_window(const E *src, int64_t e0, int n, E *ds
   static_assert(R * RB == CAP * static_cast<int>(sizeof(E)), "whole rounds per window");
   static_assert(A <= kWinSlack, "the shifted window fits CAP entries");
   const int sh = static_cast<int>(e0 & (A - 1));
-  const uint32_t need = static_cast<uint32_t>((sh + n) * sizeof(E) + D - 1) / D * D;
+  const uint32_t need =
+      n > 0 ? static_cast<uint32_t>((sh + n) * sizeof(E) + D - 1) / D * D : 0u;
   const rsrc_t r = make_rsrc(src + (e0 - sh), need);
   char *base = reinterpret_cast<char *>(dst) + (tid >> 6) * 64 * D;
 #pragma unroll
@@ -671,14 +737,87 @@ __device__ __forceinline__ int lds_window(const E *src, int64_t e0, int n, E *ds
   return sh;
 }
 
+// Row offsets 0..nr of a block into registers: every lane's loads issued
+// together (indices clamped to valid rows; row_begin without its branch).
+template <int WG, int NI, typename TI>
+__device__ __forceinline__ void rows_offsets_load(const FwdArgs &a, int64_t r0, int64_t nr,
+                                                  TI (&raw)[NI], int tid) {
+  const TI *rowptr = static_cast<const TI *>(a.rowptr);
+#pragma unroll
+  for (int k = 0; k < NI; k++) {
+    if (k * WG > nr) break;
+    const int64_t i = tid + k * WG;
+    const int64_t r = r0 + (i <= nr ? i : nr);
+    raw[k] = rowptr[r < a.nrows || a.sent ? r : r - 1];
+  }
+}
+
+template <int WG, int NI, typename TI>
+__device__ __forceinline__ void rows_offsets_store(const FwdArgs &a, int64_t r0, int64_t e0,
+                                                   int64_t nr, const TI (&raw)[NI], int *rp_s,
+                                                   int tid) {
+#pragma unroll
+  for (int k = 0; k < NI; k++) {
+    if (k * WG > nr) break;
+    const int64_t i = tid + k * WG;
+    if (i <= nr) {
+      const int64_t r = r0 + i;
+      int64_t v0 = a.nnz;
+      if (r < a.nrows || a.sent) {
+        v0 = static_cast<int64_t>(raw[k]) - a.base;
+        v0 = v0 < 0 ? 0 : (v0 > a.nnz ? a.nnz : v0);
+      }
+      rp_s[i] = static_cast<int>(v0 - e0);
+    }
+  }
+}
+
+// A last row [es, e1) longer than the window (row r1 - 1): the whole
+// workgroup, strided coalesced loads, in the sum order of process_block's
+// long-row path (the same bits).
+template <int WG, typename TV, typename TX, typename TY, typename TA, typename TI>
+__device__ __forceinline__ void rows_long_last(const FwdArgs &a, int64_t es, int64_t e1,
+                                               int64_t r1, TA *red, int tid) {
+  const TI *colind = static_cast<const TI *>(a.colind);
+  const TV *vals = static_cast<const TV *>(a.vals);
+  const TX *x = static_cast<const TX *>(a.x);
+  TY *y = static_cast<TY *>(a.y);
+  const TA alpha = static_cast<TA>(a.alpha), beta = static_cast<TA>(a.beta);
+  const int64_t last = r1 - 1;
+  TA acc0 = 0, acc1 = 0;
+  int64_t q = es + tid;
+  for (; q + 3 * WG < e1; q += 4 * WG) {
+    const TV w0 = ld_stream(vals + q), w1 = ld_stream(vals + q + WG);
+    const TV w2 = ld_stream(vals + q + 2 * WG), w3 = ld_stream(vals + q + 3 * WG);
+    const TI c0 = ld_stream(colind + q), c1 = ld_stream(colind + q + WG);
+    const TI c2 = ld_stream(colind + q + 2 * WG), c3 = ld_stream(colind + q + 3 * WG);
+    const TA x0 = to_acc<TA>(x[static_cast<int64_t>(c0)]);
+    const TA x1 = to_acc<TA>(x[static_cast<int64_t>(c1)]);
+    const TA x2 = to_acc<TA>(x[static_cast<int64_t>(c2)]);
+    const TA x3 = to_acc<TA>(x[static_cast<int64_t>(c3)]);
+    acc0 += to_acc<TA>(w0) * x0;
+    acc1 += to_acc<TA>(w1) * x1;
+    acc0 += to_acc<TA>(w2) * x2;
+    acc1 += to_acc<TA>(w3) * x3;
+  }
+  for (; q < e1; q += WG)
+    acc0 += to_acc<TA>(ld_stream(vals + q)) *
+            to_acc<TA>(x[static_cast<int64_t>(ld_stream(colind + q))]);
+  const TA s = block_sum<WG>(acc0 + acc1, red);
+  if (tid == 0) {
+    TA out = alpha * s;
+    if (beta != TA(0)) out += beta * to_acc<TA>(y[last]);
+    y[last] = from_acc<TY>(out);
+  }
+}
+
 template <int WG, int CAP, typename TV, typename TX, typename TY, typename TA, typename TI>
 __device__ __forceinline__ void process_block_rows(const FwdArgs &a, int64_t blk,
                                                    RowsSmem<WG, CAP, TV, TI, TA> &sm, int tid) {
   constexpr int ROWCAP = RowsSmem<WG, CAP, TV, TI, TA>::ROWCAP;
-  const TI *rowptr = static_cast<const TI *>(a.rowptr);
+  constexpr int NI = (ROWCAP + WG) / WG;
   const TI *colind = static_cast<const TI *>(a.colind);
   const TV *vals = static_cast<const TV *>(a.vals);
-  const TX *x = static_cast<const TX *>(a.x);
   TY *y = static_cast<TY *>(a.y);
   const TA alpha = static_cast<TA>(a.alpha), beta = static_cast<TA>(a.beta);
   const BlockRec b = a.rec[blk];
@@ -690,67 +829,16 @@ __device__ __forceinline__ void process_block_rows(const FwdArgs &a, int64_t blk
   const int shv = lds_window<WG, CAP>(vals, b.e0, n_s, sm.v, tid);
   const int shc = lds_window<WG, CAP>(colind, b.e0, n_s, sm.c, tid);
   if (rows_lds) {
-    // Row offsets 0..nr: every lane's loads issued before the first wait
-    // (indices clamped to valid rows; row_begin without its branch).
-    constexpr int NI = (ROWCAP + WG) / WG;
     TI raw[NI];
-#pragma unroll
-    for (int k = 0; k < NI; k++) {
-      if (k * WG > nr) break;
-      const int64_t i = tid + k * WG;
-      const int64_t r = b.r0 + (i <= nr ? i : nr);
-      raw[k] = rowptr[r < a.nrows || a.sent ? r : r - 1];
-    }
-#pragma unroll
-    for (int k = 0; k < NI; k++) {
-      if (k * WG > nr) break;
-      const int64_t i = tid + k * WG;
-      if (i <= nr) {
-        const int64_t r = b.r0 + i;
-        int64_t v0 = a.nnz;
-        if (r < a.nrows || a.sent) {
-          v0 = static_cast<int64_t>(raw[k]) - a.base;
-          v0 = v0 < 0 ? 0 : (v0 > a.nnz ? a.nnz : v0);
-        }
-        sm.rp_s[i] = static_cast<int>(v0 - b.e0);
-      }
-    }
+    rows_offsets_load<WG, NI>(a, b.r0, nr, raw, tid);
+    rows_offsets_store<WG, NI>(a, b.r0, b.e0, nr, raw, sm.rp_s, tid);
   }
   // the LDS copies count as vector-memory loads
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
-  row_gather_sums<WG, 8, TX, TY, TA, TI, TV, TI>(sm.v + shv, sm.c + shc, sm.rp_s, rows_lds, a,
-                                                 b.r0, b.e0, nr, y, alpha, beta);
-  if (b.es < e1) {
-    // Long last row: the whole workgroup, strided coalesced loads.
-    const int64_t last = r1 - 1;
-    // (the sum order of process_block's long-row path: the same bits)
-    TA acc0 = 0, acc1 = 0;
-    int64_t q = b.es + tid;
-    for (; q + 3 * WG < e1; q += 4 * WG) {
-      const TV w0 = ld_stream(vals + q), w1 = ld_stream(vals + q + WG);
-      const TV w2 = ld_stream(vals + q + 2 * WG), w3 = ld_stream(vals + q + 3 * WG);
-      const TI c0 = ld_stream(colind + q), c1 = ld_stream(colind + q + WG);
-      const TI c2 = ld_stream(colind + q + 2 * WG), c3 = ld_stream(colind + q + 3 * WG);
-      const TA x0 = to_acc<TA>(x[static_cast<int64_t>(c0)]);
-      const TA x1 = to_acc<TA>(x[static_cast<int64_t>(c1)]);
-      const TA x2 = to_acc<TA>(x[static_cast<int64_t>(c2)]);
-      const TA x3 = to_acc<TA>(x[static_cast<int64_t>(c3)]);
-      acc0 += to_acc<TA>(w0) * x0;
-      acc1 += to_acc<TA>(w1) * x1;
-      acc0 += to_acc<TA>(w2) * x2;
-      acc1 += to_acc<TA>(w3) * x3;
-    }
-    for (; q < e1; q += WG)
-      acc0 += to_acc<TA>(ld_stream(vals + q)) *
-              to_acc<TA>(x[static_cast<int64_t>(ld_stream(colind + q))]);
-    const TA s = block_sum<WG>(acc0 + acc1, sm.red);
-    if (tid == 0) {
-      TA out = alpha * s;
-      if (beta != TA(0)) out += beta * to_acc<TA>(y[last]);
-      y[last] = from_acc<TY>(out);
-    }
-  }
+  row_sums_any<WG, TX, TY, TA, TI>(n_s, sm.v + shv, sm.c + shc, sm.rp_s, rows_lds, a, b.r0, b.e0,
+                                   nr, y, alpha, beta);
+  if (b.es < e1) rows_long_last<WG, TV, TX, TY, TA, TI>(a, b.es, e1, r1, sm.red, tid);
   __syncthreads();  // the next block overwrites the LDS window
 }
 

@@ -309,18 +309,20 @@ static void run_spmv_impl(const Plan &p, int vtype, const void *colind,
   // gathers beat it on bands (+-16 FP32: 241 vs 286 us per call).
   if constexpr (MODE == kPlain) {
     if (k == 1 && bufx && p.nnz > 0) a.xstage = tuned_xstage() == 2;
-    // Row-ordered gathers for shares of short rows (longest at most 16
-    // entries, at least 4 on average; knob rowgather), unless x staging is
-    // forced. Measured (us per call, against entry-ordered gathers): 3-D
-    // Laplacian 256^3 FP64 391 -> 381, FP32 223 -> 212, BF16 201 -> 173;
-    // 2-D Laplacian 1000^2 FP64 21.5 -> 21.1; random columns, 4..12 entries
-    // per row, 16M rows, FP32 2362 -> 2350; band +-16 (33 per row, not
-    // picked) FP32 220 -> 226.
+    // Row-ordered gathers for shares of rows of at most 128 entries (at
+    // least 4 on average; knob rowgather), unless x staging is forced.
+    // Measured with the window copied into LDS (us per call, against
+    // entry-ordered gathers, one run): 3-D Laplacian 256^3 FP64 397 -> 355,
+    // FP32 227 -> 210, BF16 201 -> 167; 2-D Laplacian 1000^2 FP64 21.6 ->
+    // 19.9; band +-16 FP32 217 -> 210, FP64 350 -> 321, BF16 194 -> 179;
+    // band +-40 FP32 283 -> 258; random columns, 16M rows of 4..12 entries
+    // FP32 2369 -> 2379, 4M rows of 16..48 FP32 9130 -> 9168, BF16 8127 ->
+    // 7947, 2M rows of 40..64 FP64 16143 -> 15813.
     if (k == 1 && bufx && p.nnz > 0) {
       const int rg = tuned_rowgather();
       a.rowgather = a.xstage ? 0
                     : rg >= 2 ? rg
-                    : (rg == 1 && p.lmax > 0 && p.lmax <= 16 && p.nnz >= 4 * p.nrows) ? 1
+                    : (rg == 1 && p.lmax > 0 && p.lmax <= 128 && p.nnz >= 4 * p.nrows) ? 1
                                                                                      : 0;
     }
   }

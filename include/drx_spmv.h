@@ -117,9 +117,11 @@ const char *drx_last_error(void);
  *                 side: each block's values and columns are staged in LDS
  *                 and each lane sums one row, gathering its x entries in
  *                 row order (stencils: a wave's gather reads 64 consecutive
- *                 rows' neighbours). 1 (default) for plans whose longest
- *                 row has at most 16 entries and whose rows average at
- *                 least 4 (2048-entry windows), copying the window straight
+ *                 rows' neighbours; lane groups of 8 or 16 per row in
+ *                 blocks of longer rows). 1 (default) for plans whose
+ *                 longest row has at most 128 entries and whose rows
+ *                 average at least 4 (2048-entry windows, 4096 for 16-bit
+ *                 values), copying the window straight
  *                 into LDS where vals and colind are aligned to 16 bytes (4
  *                 for 16-bit values), else through registers; 2 always,
  *                 through registers; 3 always, straight copies where

@@ -250,8 +250,9 @@ def test_autograd_gradcheck_small():
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_adjoint_with_stored_transposed_values(orc, dtype):
     """Values stored in CSC order give the same adjoint, bit for bit, as
-    reading them through the permutation (on the same 1024-entry blocks:
-    the stored-values FP32 path otherwise streams 2048-entry blocks, checked
+    reading them through the permutation (on the same 1024-entry blocks and
+    with entry-ordered gathers: by default the stored-values path streams
+    2048-entry blocks for FP32 and sums short rows in row order, checked
     against the oracle); changing the values in place (new tensor version)
     falls back to the permutation."""
     A = rand_csr(5000, 4000, 0.002, 21)
@@ -261,7 +262,7 @@ def test_adjoint_with_stored_transposed_values(orc, dtype):
     via_perm = Ad.rmatvec(dy)
     Ad.set_transposed_values()
     assert Ad.stored_transposed_values(Ad.vals)
-    with drx.tuned(wide=0):
+    with drx.tuned(wide=0, rowgather=0):
         stored = Ad.rmatvec(dy)
     stored_wide = Ad.rmatvec(dy)
     torch.cuda.synchronize()

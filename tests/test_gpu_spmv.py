@@ -784,7 +784,7 @@ def _short_rows(m, n, seed, lmax, local, empty=0.15, extra=()):
 
 def _row_gather_cases():
     """(name, rowptr, colind, vals, shape, auto): `auto` says whether the
-    default picks row-ordered gathers (longest row <= 16, >= 4 per row)."""
+    default picks row-ordered gathers (longest row <= 128, >= 4 per row)."""
     out = []
     L = drx.gen.laplacian(3, 22, vdtype=torch.float64)
     out.append(("lap3d", L.rowptr.cpu().numpy(), L.colind.cpu().numpy(), L.vals.cpu().numpy(),
@@ -793,6 +793,12 @@ def _row_gather_cases():
     out.append(("short_local", rp, ci, v, (m, 30000), True))
     rp, ci, v, m = _short_rows(20000, 50000, 2, 12, False, empty=0.05)
     out.append(("short_random", rp, ci, v, (m, 50000), True))
+    # rows of 17..64 entries: lane groups of 8 and 16 per row
+    B = drx.gen.band(30000, 16, 16, vdtype=torch.float64, seed=5)
+    out.append(("band16", B.rowptr.cpu().numpy(), B.colind.cpu().numpy(), B.vals.cpu().numpy(),
+                B.shape, True))
+    rp, ci, v, m = _short_rows(8000, 40000, 6, 64, False, empty=0.02)
+    out.append(("random_to_64", rp, ci, v, (m, 40000), True))
     # rows longer than eight entries (several rounds per lane), longer than
     # the window (the whole-workgroup path) and empty: forced only
     rp, ci, v, m = _short_rows(6000, 30000, 3, 40, True, extra=(9000, 3, 0, 2500, 17))
@@ -811,7 +817,8 @@ def test_row_ordered_gathers(orc, vdt, itype):
     staged in LDS, one lane per row) against the oracle, with alpha and beta:
     stencils, short local and random rows, rows longer than eight entries
     and than the window, empty rows, blocks with more rows than the LDS
-    holds. The window copied straight into LDS (3) and through registers
+    holds; lane groups of 8 and 16 per row for blocks of longer rows. The
+    window copied straight into LDS (3) and through registers
     (2: groups of four entries per lane, or single entries when vals start
     one element into a larger buffer) give the same bits. The default (1)
     picks them exactly for the eligible shares: the straight copy for
