@@ -288,11 +288,13 @@ def test_dvals_grouped_entries_bitexact(orc, beta):
     """FP32 dvals over 4096-entry windows with groups of four entries per
     lane (16-byte colind loads and dvals stores; vec 0 or 4) give the same
     bits as one entry per lane (vec 1), with windows starting at every
-    offset inside a group, long and empty rows, and output buffers starting
-    one to three elements into a larger allocation (single entries); beta
-    reads the old values."""
+    offset inside a group, long and empty rows, runs of short rows (blocks
+    of more rows than lanes, and of more than the LDS holds), and output
+    buffers starting one to three elements into a larger allocation (single
+    entries); beta reads the old values."""
     rng = np.random.default_rng(51)
-    lens = [0] * 7 + list(rng.integers(16, 70, 4000)) + [9000, 0, 3, 20000, 17, 1]
+    lens = ([0] * 7 + list(rng.integers(16, 70, 4000)) + [9000, 0, 3, 20000, 17, 1] + [5] * 600
+            + [1] * 5000 + list(rng.integers(16, 70, 300)))
     n = 40000
     rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
     ci = np.concatenate([np.sort(rng.choice(n, L, replace=False)) for L in lens]).astype(np.int64)
