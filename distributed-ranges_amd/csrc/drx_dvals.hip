@@ -13,6 +13,7 @@
 #include <type_traits>
 
 #include "drx_common.hpp"
+#include "drx_fwd.hpp"
 #include "drx_internal.hpp"
 
 namespace drx {
@@ -254,6 +255,7 @@ struct FusedArgs {
   const void *colind, *vals, *x, *t;
   void *y, *r, *dvals;
   int xstage;  // stage each block's x window in LDS
+  uint32_t xbytes;  // bytes of x (buffer gathers of the row-ordered kernel)
 };
 
 // Rows [ra, rb) of a block: row offsets (relative to e0) and targets from
@@ -407,8 +409,17 @@ __global__ __launch_bounds__(kWG) void k_fused_residual(FusedArgs a) {
           if (c[u] - lo >= xn && tid + u * kWG < n_s) xv[u] = to_acc<TA>(x[c[u]]);
       }
     } else if (n_s > 0) {
+      if constexpr (VW > 1) {
+        // Buffer reads, range-checked: the last group's indices past the
+        // block end can lie past the array's last entry (inside its
+        // aligned 16 bytes) and hold any value.
+        const rsrc_t rx = make_rsrc(x, a.xbytes);
 #pragma unroll
-      for (int u = 0; u < T; u++) xv[u] = to_acc<TA>(x[c[u]]);
+        for (int u = 0; u < T; u++) xv[u] = to_acc<TA>(buf_load<TX>(rx, c[u] * sizeof(TX)));
+      } else {
+#pragma unroll
+        for (int u = 0; u < T; u++) xv[u] = to_acc<TA>(x[c[u]]);
+      }
     } else {
 #pragma unroll
       for (int u = 0; u < T; u++) xv[u] = 0;
@@ -453,6 +464,192 @@ __global__ __launch_bounds__(kWG) void k_fused_residual(FusedArgs a) {
         dv[q] = from_acc<TY>(rr * to_acc<TA>(x[static_cast<int64_t>(colind[q])]));
       __syncthreads();
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row-ordered fused pass (shares of rows of at most 128 entries, the window
+// copied straight into LDS as in k_spmv_rows): G lanes per row sum it in
+// entry order (the forward kernel's order, so y equals its bits), the
+// group's lane 0 writes y and r = y - t, and each lane writes the dvals of
+// its own entries, r * x, from the x values it gathered (rows longer than
+// one round gather x again for the later rounds).
+// ---------------------------------------------------------------------------
+template <int WG, int G, int U, typename TV, typename TX, typename TY, typename TA, typename TI>
+__device__ __forceinline__ void fused_row_pass(const TV *v_s, const TI *c_s, const int *rp_s,
+                                               bool rows_lds, const FusedArgs &a, int64_t r0,
+                                               int64_t e0, int64_t nr) {
+  const TI *rowptr = static_cast<const TI *>(a.rowptr);
+  const TY *t = static_cast<const TY *>(a.t);
+  TY *y = static_cast<TY *>(a.y);
+  TY *res = static_cast<TY *>(a.r);
+  TY *dv = static_cast<TY *>(a.dvals);
+  const rsrc_t rx = make_rsrc(a.x, a.xbytes);
+  const int lane = threadIdx.x & (G - 1);
+  for (int64_t i = threadIdx.x / G; i < nr; i += WG / G) {
+    int s, e;
+    if (rows_lds) {
+      s = rp_s[i];
+      e = rp_s[i + 1];
+    } else {
+      s = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r0 + i) - e0);
+      e = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r0 + i + 1) - e0);
+    }
+    const TA tv = to_acc<TA>(t[r0 + i]);
+    TA acc = 0;
+    TA x0[U];
+    const int q0 = s + lane;
+    if (q0 < e) {
+      uint32_t cc[U];
+      TA vv[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int qq = q0 + u * G < e ? q0 + u * G : q0;
+        cc[u] = col32(c_s[qq]);
+        vv[u] = to_acc<TA>(v_s[qq]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) x0[u] = to_acc<TA>(buf_load<TX>(rx, cc[u] * sizeof(TX)));
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (q0 + u * G < e) acc += vv[u] * x0[u];
+    }
+    for (int q = q0 + G * U; q < e; q += G * U) {
+      uint32_t cc[U];
+      TA vv[U], xx[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int qq = q + u * G < e ? q + u * G : q;
+        cc[u] = col32(c_s[qq]);
+        vv[u] = to_acc<TA>(v_s[qq]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) xx[u] = to_acc<TA>(buf_load<TX>(rx, cc[u] * sizeof(TX)));
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (q + u * G < e) acc += vv[u] * xx[u];
+    }
+    acc = group_sum<G>(acc);
+    const TA rr = acc - tv;
+    if (lane == 0) {
+      y[r0 + i] = from_acc<TY>(acc);
+      res[r0 + i] = from_acc<TY>(rr);
+    }
+    if (q0 < e) {
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (q0 + u * G < e) dv[e0 + q0 + u * G] = from_acc<TY>(rr * x0[u]);
+    }
+    for (int q = q0 + G * U; q < e; q += G * U) {
+      uint32_t cc[U];
+      TA xx[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) cc[u] = col32(c_s[q + u * G < e ? q + u * G : q]);
+#pragma unroll
+      for (int u = 0; u < U; u++) xx[u] = to_acc<TA>(buf_load<TX>(rx, cc[u] * sizeof(TX)));
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (q + u * G < e) dv[e0 + q + u * G] = from_acc<TY>(rr * xx[u]);
+    }
+  }
+}
+
+template <int WG, int CAP, typename TV, typename TI, typename TA> struct FusedRowsSmem {
+  static constexpr int ROWCAP = CAP / 4;
+  alignas(16) TV v[CAP];
+  alignas(16) TI c[CAP];
+  int rp_s[ROWCAP + 1];
+  TA red[WG / 64];
+};
+
+// As many workgroups as the LDS holds, at most 7 waves (6 for 16-bit
+// values, 5 for 8-byte sums): the x values a lane keeps for its dvals spill
+// registers at one wave more.
+template <int WG, int CAP, typename TV, typename TI, typename TA> constexpr int fused_rows_waves() {
+  constexpr int lds = static_cast<int>(sizeof(FusedRowsSmem<WG, CAP, TV, TI, TA>));
+  constexpr int w = (160 * 1024) / lds * (WG / 256);
+  constexpr int most = sizeof(TA) == 8 ? 5 : (sizeof(TV) == 2 ? 6 : 7);
+  return w > most ? most : (w < 1 ? 1 : w);
+}
+
+template <int WG, int CAP, typename TV, typename TX, typename TY, typename TA, typename TI>
+__global__ __launch_bounds__(WG)
+__attribute__((amdgpu_waves_per_eu(fused_rows_waves<WG, CAP, TV, TI, TA>(), 8)))
+void k_fused_rows(FusedArgs a) {
+  using Sm = FusedRowsSmem<WG, CAP, TV, TI, TA>;
+  constexpr int ROWCAP = Sm::ROWCAP;
+  constexpr int NI = (ROWCAP + WG) / WG;
+  __shared__ Sm sm;
+  int tid;
+  asm volatile("" : "=v"(tid) : "0"(static_cast<int>(threadIdx.x)));
+  const TI *colind = static_cast<const TI *>(a.colind);
+  const TV *vals = static_cast<const TV *>(a.vals);
+  const TX *x = static_cast<const TX *>(a.x);
+  for (int64_t wi = blockIdx.x; wi < a.per_xcd * kXcds; wi += a.grid) {
+    const int64_t blk = map_block(wi, a.per_xcd);
+    if (blk >= a.nblocks) continue;
+    const BlockRec b = a.rec[blk];
+    const int64_t r1 = a.rec[blk + 1].r0, e1 = a.rec[blk + 1].e0;
+    if (b.r0 >= r1) continue;
+    const int n_s = static_cast<int>(b.es - b.e0);
+    const int64_t nr = (b.es == e1 ? r1 : r1 - 1) - b.r0;
+    const bool rows_lds = nr < ROWCAP;
+    const int shv = lds_window<WG, CAP>(vals, b.e0, n_s, sm.v, tid);
+    const int shc = lds_window<WG, CAP>(colind, b.e0, n_s, sm.c, tid);
+    if (rows_lds) {
+      TI raw[NI];
+      rows_offsets_load<WG, NI>(a, b.r0, nr, raw, tid);
+      rows_offsets_store<WG, NI>(a, b.r0, b.e0, nr, raw, sm.rp_s, tid);
+    }
+    // the LDS copies count as vector-memory loads
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (nr > 0) {
+      const TV *vs = sm.v + shv;
+      const TI *cs = sm.c + shc;
+      const int64_t avg = n_s / nr;
+      if (avg <= 12)
+        fused_row_pass<WG, 1, 8, TV, TX, TY, TA, TI>(vs, cs, sm.rp_s, rows_lds, a, b.r0, b.e0, nr);
+      else if (avg <= 40)
+        fused_row_pass<WG, 8, 5, TV, TX, TY, TA, TI>(vs, cs, sm.rp_s, rows_lds, a, b.r0, b.e0, nr);
+      else
+        fused_row_pass<WG, 16, 5, TV, TX, TY, TA, TI>(vs, cs, sm.rp_s, rows_lds, a, b.r0, b.e0,
+                                                      nr);
+    }
+    if (b.es < e1) {
+      // Long last row: the whole workgroup (the sum order of the forward
+      // kernels' long-row path).
+      const int64_t last = r1 - 1;
+      TA acc0 = 0, acc1 = 0;
+      int64_t q = b.es + tid;
+      for (; q + 3 * WG < e1; q += 4 * WG) {
+        const TV w0 = ld_stream(vals + q), w1 = ld_stream(vals + q + WG);
+        const TV w2 = ld_stream(vals + q + 2 * WG), w3 = ld_stream(vals + q + 3 * WG);
+        const TI c0 = ld_stream(colind + q), c1 = ld_stream(colind + q + WG);
+        const TI c2 = ld_stream(colind + q + 2 * WG), c3 = ld_stream(colind + q + 3 * WG);
+        const TA x0 = to_acc<TA>(x[static_cast<int64_t>(c0)]);
+        const TA x1 = to_acc<TA>(x[static_cast<int64_t>(c1)]);
+        const TA x2 = to_acc<TA>(x[static_cast<int64_t>(c2)]);
+        const TA x3 = to_acc<TA>(x[static_cast<int64_t>(c3)]);
+        acc0 += to_acc<TA>(w0) * x0;
+        acc1 += to_acc<TA>(w1) * x1;
+        acc0 += to_acc<TA>(w2) * x2;
+        acc1 += to_acc<TA>(w3) * x3;
+      }
+      for (; q < e1; q += WG)
+        acc0 += to_acc<TA>(ld_stream(vals + q)) *
+                to_acc<TA>(x[static_cast<int64_t>(ld_stream(colind + q))]);
+      const TA sum = block_sum<WG>(acc0 + acc1, sm.red);
+      const TA rr = sum - to_acc<TA>(static_cast<const TY *>(a.t)[last]);
+      if (tid == 0) {
+        static_cast<TY *>(a.y)[last] = from_acc<TY>(sum);
+        static_cast<TY *>(a.r)[last] = from_acc<TY>(rr);
+      }
+      TY *dv = static_cast<TY *>(a.dvals);
+      for (int64_t q2 = b.es + tid; q2 < e1; q2 += WG)
+        dv[q2] = from_acc<TY>(rr * to_acc<TA>(x[static_cast<int64_t>(colind[q2])]));
+    }
+    __syncthreads();  // the next block overwrites the LDS window
   }
 }
 
@@ -591,10 +788,50 @@ void run_fused(const Plan &p, int vtype, const void *colind, const void *vals,
   const dim3 grid(static_cast<unsigned>(a.grid)), block(kWG);
   const bool i32 = p.itype == DRX_I32;
   const bool small = p.cap == 1024;
+  // Row-ordered pass with the window copied straight into LDS only when
+  // knob rowgather is 3: it measured slower than the entry-ordered kernel
+  // (band +-16 FP32 374 vs 353 us per call, the same box class), its
+  // row-ordered dvals stores reaching memory in 32-byte pieces.
+  {
+    const int rg = tuned_rowgather();
+    const uint64_t xb = static_cast<uint64_t>(std::max<int64_t>(p.ncols, 1)) * dtype_size(xtype);
+    const uintptr_t vw = dtype_size(vtype) >= 4 ? 16 : 4;
+    const bool rows = p.nnz > 0 && !a.xstage && p.cap == 2048 && xb < (uint64_t(1) << 32) &&
+                      rg == 3 &&
+                      reinterpret_cast<uintptr_t>(colind) % 16 == 0 &&
+                      reinterpret_cast<uintptr_t>(vals) % vw == 0;
+    if (rows) {
+      a.xbytes = static_cast<uint32_t>(xb);
+#define DRX_FR(TV, TX, TY, TA)                                                               \
+  do {                                                                                       \
+    if (i32)                                                                                 \
+      hipLaunchKernelGGL((k_fused_rows<256, 2048, TV, TX, TY, TA, int32_t>), grid, block, 0, s, a); \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_fused_rows<256, 2048, TV, TX, TY, TA, int64_t>), grid, block, 0, s, a); \
+  } while (0)
+      if (vtype == DRX_F64 && xtype == DRX_F64 && ytype == DRX_F64)
+        DRX_FR(double, double, double, double);
+      else if (vtype == DRX_F32 && xtype == DRX_F32 && ytype == DRX_F32)
+        DRX_FR(float, float, float, float);
+      else if (vtype == DRX_BF16 && xtype == DRX_BF16 && ytype == DRX_F32)
+        DRX_FR(bf16, bf16, float, float);
+      else if (vtype == DRX_F16 && xtype == DRX_F16 && ytype == DRX_F32)
+        DRX_FR(f16, f16, float, float);
+      else
+        throw Error(DRX_ERR_UNSUPPORTED, "unsupported (vtype, xtype, ytype)");
+#undef DRX_FR
+      DRX_HIP(hipGetLastError());
+      return;
+    }
+  }
   // groups of four entries per lane: 32-bit indices, vals / colind aligned
-  // to a group, no x staging (knob vec 0 or 4)
+  // to a group, no x staging (knob vec 0 or 4), x within 4 GB (its reads
+  // are range-checked buffer loads)
   const int vec = tuned_vec();
+  const uint64_t xbytes = static_cast<uint64_t>(std::max<int64_t>(p.ncols, 1)) * dtype_size(xtype);
+  if (xbytes < (uint64_t(1) << 32)) a.xbytes = static_cast<uint32_t>(xbytes);
   const bool g4 = i32 && !a.xstage && (vec == 0 || vec == 4) && dtype_size(vtype) <= 4 &&
+                  xbytes < (uint64_t(1) << 32) &&
                   reinterpret_cast<uintptr_t>(colind) % 16 == 0 &&
                   reinterpret_cast<uintptr_t>(vals) % (4 * dtype_size(vtype)) == 0;
 #define DRX_FU_X(TV, TX, TY, TA, XS)                                           \

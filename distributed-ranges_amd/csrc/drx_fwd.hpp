@@ -739,8 +739,8 @@ __device__ __forceinline__ int lds_window(const E *src, int64_t e0, int n, E *ds
 
 // Row offsets 0..nr of a block into registers: every lane's loads issued
 // together (indices clamped to valid rows; row_begin without its branch).
-template <int WG, int NI, typename TI>
-__device__ __forceinline__ void rows_offsets_load(const FwdArgs &a, int64_t r0, int64_t nr,
+template <int WG, int NI, typename TI, typename Args>
+__device__ __forceinline__ void rows_offsets_load(const Args &a, int64_t r0, int64_t nr,
                                                   TI (&raw)[NI], int tid) {
   const TI *rowptr = static_cast<const TI *>(a.rowptr);
 #pragma unroll
@@ -752,8 +752,8 @@ __device__ __forceinline__ void rows_offsets_load(const FwdArgs &a, int64_t r0, 
   }
 }
 
-template <int WG, int NI, typename TI>
-__device__ __forceinline__ void rows_offsets_store(const FwdArgs &a, int64_t r0, int64_t e0,
+template <int WG, int NI, typename TI, typename Args>
+__device__ __forceinline__ void rows_offsets_store(const Args &a, int64_t r0, int64_t e0,
                                                    int64_t nr, const TI (&raw)[NI], int *rp_s,
                                                    int tid) {
 #pragma unroll

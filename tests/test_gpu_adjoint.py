@@ -174,17 +174,20 @@ def test_fused_residual_x_staging_bitexact(orc, dtype, kind):
     x = drx.gen.uniform(A.shape[1], dtype, -1, 1, seed=34)
     t = drx.gen.uniform(A.shape[0], dtype, -1, 1, seed=35)
     outs = {}
+    # (entry-ordered kernels throughout: rowgather 0)
     for xs in (0, 2, 1):
-        with drx.tuned(xstage=xs, vec=1):
+        with drx.tuned(xstage=xs, vec=1, rowgather=0):
             outs[xs] = A.fused_residual(x, t)
     # groups of four entries per lane (FP32, default vec) and single entries
     # when vals start one element into a larger buffer
-    outs["g4"] = A.fused_residual(x, t)
+    with drx.tuned(rowgather=0):
+        outs["g4"] = A.fused_residual(x, t)
     buf = torch.zeros(A.nnz + 1, dtype=dtype, device=DEV)
     buf[1:] = A.vals
     v0 = A.vals
     A.vals = buf[1:]
-    outs["off"] = A.fused_residual(x, t)
+    with drx.tuned(rowgather=0):
+        outs["off"] = A.fused_residual(x, t)
     A.vals = v0
     torch.cuda.synchronize()
     for xs in (2, 1, "g4", "off"):
@@ -337,3 +340,73 @@ def test_dvals_fp32_wide_blocks(orc, wide):
         d = Ad.dvals(torch.as_tensor(x).to(DEV), torch.as_tensor(dy).to(DEV))
     ref = orc.dvals(rp, ci, x, dy)
     assert np.abs(d.cpu().numpy() - ref).max() <= 1e-6
+
+
+def _fused_rows_cases():
+    """(name, rowptr, colind, vals, shape, eligible): eligible shares are
+    the ones the forward product sums in row order by default (longest row
+    <= 128, >= 4 per row)."""
+    rng = np.random.default_rng(61)
+    out = []
+    B = drx.gen.band(40000, 16, 16, vdtype=torch.float64, seed=62)
+    out.append(("band16", B.rowptr.cpu().numpy(), B.colind.cpu().numpy(), B.vals.cpu().numpy(),
+                B.shape, True))
+    L = drx.gen.laplacian(3, 20, vdtype=torch.float64)
+    out.append(("lap3d", L.rowptr.cpu().numpy(), L.colind.cpu().numpy(), L.vals.cpu().numpy(),
+                L.shape, True))
+    n = 30000
+    for name, lens, auto in (
+            ("rows_to_120", list(rng.integers(0, 121, 3000)), True),
+            # rows longer than one round, than the window, empty; tiny rows
+            ("long_rows", list(rng.integers(0, 200, 2000)) + [9000, 0, 3, 2500, 17], False),
+            ("tiny_rows", list(rng.integers(0, 3, 20000)), False)):
+        rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        ci = np.concatenate([np.sort(rng.choice(n, L_, replace=False)) for L_ in lens]).astype(np.int64)
+        out.append((name, rp, ci, rng.uniform(-1, 1, ci.size), (len(lens), n), auto))
+    return out
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("itype", [torch.int32, torch.int64])
+def test_fused_residual_row_ordered(orc, dtype, itype):
+    """The row-ordered fused kernel (rowgather 3: window copied into LDS,
+    lane groups per row, each lane writing the dvals of its own entries): y
+    equals the forward product's bits under the same knob (the same
+    row-ordered sums), r = y - t exactly, dvals within the oracle's bound.
+    The default keeps the entry-ordered fused kernel (same bits as
+    rowgather 0), also run on these shares (nnz not a multiple of four:
+    groups of four read indices past the arrays' end)."""
+    low = dtype == torch.bfloat16
+    sdt = np.float64 if dtype == torch.float64 else np.float32
+    for name, rp, ci, v, shape, _ in _fused_rows_cases():
+        m, n = shape
+        A = drx.from_host(rp, ci, v.astype(sdt), shape, itype=itype)
+        if low:
+            A.vals = A.vals.to(dtype)
+        xdt = torch.float64 if dtype == torch.float64 else torch.float32
+        x = drx.gen.uniform(n, xdt, -1, 1, seed=63).to(dtype)
+        t = drx.gen.uniform(m, xdt, -1, 1, seed=64)
+        res = {}
+        for rg in (3, 1, 0):
+            with drx.tuned(rowgather=rg):
+                res[rg] = A.fused_residual(x, t)
+                if rg == 3:
+                    yf = A.matvec(x)
+        torch.cuda.synchronize()
+        y, r, dv = res[3]
+        assert torch.equal(y, yf), name
+        assert torch.equal(r, y - t), name
+        for a_, b_ in zip(res[1], res[0]):
+            assert torch.equal(a_, b_), name
+        y0, r0, dv0 = res[0]
+        assert torch.equal(r0, y0 - t), name
+        vd = A.vals.double().cpu().numpy()
+        xd = x.double().cpu().numpy()
+        ystar, absy = orc.spmv(rp, ci, vd, xd)
+        tol = 1e-12 if dtype == torch.float64 else 1e-5
+        for yy, rr, dd in (res[3], res[0]):
+            assert componentwise_err(yy.double().cpu().numpy(), ystar, absy) < tol, name
+            ref = orc.dvals(rp, ci, xd, rr.double().cpu().numpy())
+            bound = (1e-15 if dtype == torch.float64 else 1e-6) * max(np.abs(ref).max(), 1e-300)
+            assert np.abs(dd.double().cpu().numpy() - ref).max() <= bound, name
+        A.close()
