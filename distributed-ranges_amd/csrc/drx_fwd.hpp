@@ -47,6 +47,7 @@ struct FwdArgs {
   int gshift;  // lane-group choice uses (mean row length >> gshift)
   int xstage;  // kPlain, k = 1: stage each block's x window in LDS
   int rowgather;  // kPlain, k = 1: row-ordered gathers (kernels built with RG)
+  int64_t lmax;   // longest row of the share (-1 unknown): row-ordered round size
   const void *xr;  // k >= 2: x in row-major chunks of KP columns
   int64_t ncols;   // columns of the share (rows of one xr chunk)
 };
@@ -252,7 +253,16 @@ __device__ __forceinline__ void row_sums_any(int n_s, const VS *v_s, const CS *c
                                              int64_t r0, int64_t e0, int64_t nr, TY *y, TA alpha,
                                              TA beta) {
   const int64_t avg = nr > 0 ? n_s / nr : 0;
-  if (avg <= 12)
+  // one lane per row: rounds of the share's longest row when that is 5 or
+  // 7 (2-D and 3-D stencils), so that no gather slot of a round is idle;
+  // the sums are the same either way (entry order)
+  if (avg <= 12 && a.lmax > 0 && a.lmax <= 5)
+    row_gather_sums<WG, 5, TX, TY, TA, TI, VS, CS>(v_s, c_s, rp_s, rows_lds, a, r0, e0, nr, y,
+                                                   alpha, beta);
+  else if (avg <= 12 && a.lmax > 5 && a.lmax <= 7)
+    row_gather_sums<WG, 7, TX, TY, TA, TI, VS, CS>(v_s, c_s, rp_s, rows_lds, a, r0, e0, nr, y,
+                                                   alpha, beta);
+  else if (avg <= 12)
     row_gather_sums<WG, 8, TX, TY, TA, TI, VS, CS>(v_s, c_s, rp_s, rows_lds, a, r0, e0, nr, y,
                                                    alpha, beta);
   else if (avg <= 40)
@@ -842,10 +852,13 @@ __device__ __forceinline__ void process_block_rows(const FwdArgs &a, int64_t blk
   __syncthreads();  // the next block overwrites the LDS window
 }
 
+// As many workgroups as the LDS holds, at most 8 waves (7 for 8-byte sums,
+// which spill at 8).
 template <int WG, int CAP, typename TV, typename TI, typename TA> constexpr int rows_waves() {
   constexpr int lds = static_cast<int>(sizeof(RowsSmem<WG, CAP, TV, TI, TA>));
   constexpr int w = (160 * 1024) / lds * (WG / 256);
-  return w > 8 ? 8 : (w < 1 ? 1 : w);
+  constexpr int most = sizeof(TA) == 8 ? 7 : 8;
+  return w > most ? most : (w < 1 ? 1 : w);
 }
 
 template <int WG, int CAP, typename TV, typename TX, typename TY, typename TA, typename TI>

@@ -281,6 +281,7 @@ static void run_spmv_impl(const Plan &p, int vtype, const void *colind,
   a.alpha = alpha;
   a.beta = beta;
   a.gshift = 0;
+  a.lmax = p.lmax;
   const uint64_t xb =
       static_cast<uint64_t>(std::max<int64_t>(p.ncols, 1)) * dtype_size(xtype);
   const bool bufx = xb < (uint64_t(1) << 32);
