@@ -244,6 +244,160 @@ __global__ __launch_bounds__(WG) void k_dvals(DvalsArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Row-ordered dvals (one right-hand side; shares of rows of at most 128
+// entries): the block's column indices are copied straight into LDS, G lanes
+// per row gather x in row order with the row's dy in a register (no row-id
+// table, no dy staging) and put dy x in LDS at the entry's slot, and the
+// block's dvals are then written in entry order with 16-byte stores.
+// ---------------------------------------------------------------------------
+template <int WG, int CAP, typename TI, typename TA> struct DvalsRowsSmem {
+  static constexpr int ROWCAP = CAP / 4;
+  alignas(16) TI c[CAP];
+  alignas(16) TA o[CAP];
+  int rp_s[ROWCAP + 1];
+};
+
+template <int WG, int G, int U, typename TX, typename TA, typename TI>
+__device__ __forceinline__ void dvals_row_pass(const TI *c_s, TA *o_s, const int *rp_s,
+                                               bool rows_lds, const DvalsArgs &a, int64_t r0,
+                                               int64_t e0, int64_t nr) {
+  const TI *rowptr = static_cast<const TI *>(a.rowptr);
+  const TX *dy = static_cast<const TX *>(a.dy);
+  const rsrc_t rx = make_rsrc(a.x, a.xbytes);
+  const int lane = threadIdx.x & (G - 1);
+  for (int64_t i = threadIdx.x / G; i < nr; i += WG / G) {
+    int s, e;
+    if (rows_lds) {
+      s = rp_s[i];
+      e = rp_s[i + 1];
+    } else {
+      s = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r0 + i) - e0);
+      e = static_cast<int>(row_begin(rowptr, a.base, a.nrows, a.nnz, a.sent, r0 + i + 1) - e0);
+    }
+    const TA d = to_acc<TA>(dy[r0 + i]);
+    for (int q = s + lane; q < e; q += G * U) {
+      uint32_t cc[U];
+      TA xx[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) cc[u] = col32(c_s[q + u * G < e ? q + u * G : q]);
+#pragma unroll
+      for (int u = 0; u < U; u++) xx[u] = to_acc<TA>(buf_load<TX>(rx, cc[u] * sizeof(TX)));
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (q + u * G < e) o_s[q + u * G] = d * xx[u];
+    }
+  }
+}
+
+template <int WG, int CAP, typename TI, typename TA> constexpr int dvals_rows_waves() {
+  constexpr int lds = static_cast<int>(sizeof(DvalsRowsSmem<WG, CAP, TI, TA>));
+  constexpr int w = (160 * 1024) / lds * (WG / 256);
+  return w > 8 ? 8 : (w < 1 ? 1 : w);
+}
+
+// VW4: FP32 dvals aligned to 16 bytes, stored in groups of four entries
+// (the block's first and last groups entry by entry).
+template <int WG, int CAP, typename TX, typename TD, typename TA, typename TI, bool VW4>
+__global__ __launch_bounds__(WG)
+__attribute__((amdgpu_waves_per_eu(dvals_rows_waves<WG, CAP, TI, TA>(), 8)))
+void k_dvals_rows(DvalsArgs a) {
+  using Sm = DvalsRowsSmem<WG, CAP, TI, TA>;
+  constexpr int ROWCAP = Sm::ROWCAP;
+  constexpr int NI = (ROWCAP + WG) / WG;
+  static_assert(!VW4 || sizeof(TD) == 4, "groups of four 4-byte dvals");
+  __shared__ Sm sm;
+  int tid;
+  asm volatile("" : "=v"(tid) : "0"(static_cast<int>(threadIdx.x)));
+  const TI *colind = static_cast<const TI *>(a.colind);
+  const TX *x = static_cast<const TX *>(a.x);
+  const TX *dy = static_cast<const TX *>(a.dy);
+  TD *dv = static_cast<TD *>(a.dvals);
+  const TA alpha = static_cast<TA>(a.alpha), beta = static_cast<TA>(a.beta);
+  for (int64_t wi = blockIdx.x; wi < a.per_xcd * kXcds; wi += a.grid) {
+    const int64_t blk = map_block(wi, a.per_xcd);
+    if (blk >= a.nblocks) continue;
+    const BlockRec b = a.rec[blk];
+    const int64_t r1 = a.rec[blk + 1].r0, e1 = a.rec[blk + 1].e0;
+    if (b.r0 >= r1) continue;
+    const int n_s = static_cast<int>(b.es - b.e0);
+    const int64_t nr = (b.es == e1 ? r1 : r1 - 1) - b.r0;
+    const bool rows_lds = nr < ROWCAP;
+    const int shc = lds_window<WG, CAP>(colind, b.e0, n_s, sm.c, tid);
+    if (rows_lds) {
+      TI raw[NI];
+      rows_offsets_load<WG, NI>(a, b.r0, nr, raw, tid);
+      rows_offsets_store<WG, NI>(a, b.r0, b.e0, nr, raw, sm.rp_s, tid);
+    }
+    // the LDS copy counts as a vector-memory load
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    // o[slot] = dy[row] x[col] for slots [shc, shc + n_s)
+    if (nr > 0) {
+      const TI *cs = sm.c + shc;
+      TA *os = sm.o + shc;
+      const int64_t avg = n_s / nr;
+      if (avg <= 12)
+        dvals_row_pass<WG, 1, 8, TX, TA, TI>(cs, os, sm.rp_s, rows_lds, a, b.r0, b.e0, nr);
+      else if (avg <= 40)
+        dvals_row_pass<WG, 8, 5, TX, TA, TI>(cs, os, sm.rp_s, rows_lds, a, b.r0, b.e0, nr);
+      else
+        dvals_row_pass<WG, 16, 5, TX, TA, TI>(cs, os, sm.rp_s, rows_lds, a, b.r0, b.e0, nr);
+    }
+    __syncthreads();
+    if constexpr (VW4) {
+      // groups of four from the 16-byte-aligned entry at or below e0
+      const int sh4 = static_cast<int>(b.e0 & 3);
+      const int ng = (n_s + sh4 + 3) / 4;
+      for (int g = tid; g < ng; g += WG) {
+        const int q0 = g * 4 - sh4;  // block-relative entry of the group's first slot
+        if (q0 >= 0 && q0 + 4 <= n_s) {
+          TD *p = dv + b.e0 + q0;
+          Words<4> o;
+          if (beta != TA(0)) {
+            const Words<4> old = *reinterpret_cast<const Words<4> *>(p);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+              o.w[i] = __builtin_bit_cast(uint32_t, static_cast<TD>(dv_out<TA>(
+                  alpha, sm.o[shc + q0 + i], beta,
+                  static_cast<TA>(__builtin_bit_cast(TD, old.w[i])))));
+          } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+              o.w[i] = __builtin_bit_cast(uint32_t, static_cast<TD>(alpha * sm.o[shc + q0 + i]));
+          }
+          *reinterpret_cast<Words<4> *>(p) = o;
+        } else {
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            const int q = q0 + i;
+            if (q >= 0 && q < n_s) {
+              const TA old = beta != TA(0) ? static_cast<TA>(dv[b.e0 + q]) : TA(0);
+              dv[b.e0 + q] = static_cast<TD>(dv_out<TA>(alpha, sm.o[shc + q], beta, old));
+            }
+          }
+        }
+      }
+    } else {
+      for (int q = tid; q < n_s; q += WG) {
+        const TA old = beta != TA(0) ? static_cast<TA>(dv[b.e0 + q]) : TA(0);
+        dv[b.e0 + q] = static_cast<TD>(dv_out<TA>(alpha, sm.o[shc + q], beta, old));
+      }
+    }
+    if (b.es < e1) {
+      const int64_t last = r1 - 1;
+      const TA d = to_acc<TA>(dy[last]);
+      for (int64_t q = b.es + tid; q < e1; q += WG) {
+        const TA old = beta != TA(0) ? static_cast<TA>(dv[q]) : TA(0);
+        dv[q] = static_cast<TD>(
+            dv_out<TA>(alpha, d * to_acc<TA>(x[static_cast<int64_t>(ld_stream(colind + q))]),
+                       beta, old));
+      }
+    }
+    __syncthreads();  // the next block overwrites the LDS window
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Fused forward + residual + dvals (k = 1)
 // ---------------------------------------------------------------------------
 struct FusedArgs {
@@ -687,6 +841,48 @@ void run_dvals(const Plan &p, const void *colind, int xtype, const void *x,
   a.xbytes = bufx ? static_cast<uint32_t>(xb) : 0;
   const dim3 grid(static_cast<unsigned>(a.grid));
   const bool i32 = p.itype == DRX_I32;
+  // Row-ordered dvals for the shares the forward product sums in row order
+  // (knob rowgather: 1 by the plan's rows, 3 always, 0 and 2 never), one
+  // right-hand side, colind aligned to 16 bytes, the plan's 2048-entry
+  // windows.
+  {
+    const int rg = tuned_rowgather();
+    const bool rows = k == 1 && bufx && p.cap == 2048 &&
+                      reinterpret_cast<uintptr_t>(colind) % 16 == 0 &&
+                      (rg == 3 || (rg == 1 && p.lmax > 0 && p.lmax <= 128 &&
+                                   p.nnz >= 4 * p.nrows));
+    if (rows) {
+      const bool v4 = dvtype == DRX_F32 && reinterpret_cast<uintptr_t>(dvals) % 16 == 0;
+#define DRX_DR(TX, TD, TA)                                                                   \
+  do {                                                                                       \
+    if (i32 && v4)                                                                           \
+      hipLaunchKernelGGL((k_dvals_rows<256, 2048, TX, TD, TA, int32_t, std::is_same_v<TD, float>>), \
+                         grid, dim3(256), 0, s, a);                                          \
+    else if (i32)                                                                            \
+      hipLaunchKernelGGL((k_dvals_rows<256, 2048, TX, TD, TA, int32_t, false>), grid, dim3(256), \
+                         0, s, a);                                                           \
+    else if (v4)                                                                             \
+      hipLaunchKernelGGL((k_dvals_rows<256, 2048, TX, TD, TA, int64_t, std::is_same_v<TD, float>>), \
+                         grid, dim3(256), 0, s, a);                                          \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_dvals_rows<256, 2048, TX, TD, TA, int64_t, false>), grid, dim3(256), \
+                         0, s, a);                                                           \
+  } while (0)
+      if (xtype == DRX_F64 && dvtype == DRX_F64)
+        DRX_DR(double, double, double);
+      else if (xtype == DRX_F32 && dvtype == DRX_F32)
+        DRX_DR(float, float, float);
+      else if (xtype == DRX_BF16 && dvtype == DRX_F32)
+        DRX_DR(bf16, float, float);
+      else if (xtype == DRX_F16 && dvtype == DRX_F32)
+        DRX_DR(f16, float, float);
+      else
+        throw Error(DRX_ERR_UNSUPPORTED, "unsupported (xtype, dvtype)");
+#undef DRX_DR
+      DRX_HIP(hipGetLastError());
+      return;
+    }
+  }
 #define DRX_DV_I(WG, CAP, TX, TD, TA)                                          \
   do {                                                                         \
     if (i32 && bufx)                                                           \

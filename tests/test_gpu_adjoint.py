@@ -410,3 +410,39 @@ def test_fused_residual_row_ordered(orc, dtype, itype):
             bound = (1e-15 if dtype == torch.float64 else 1e-6) * max(np.abs(ref).max(), 1e-300)
             assert np.abs(dd.double().cpu().numpy() - ref).max() <= bound, name
         A.close()
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.bfloat16])
+@pytest.mark.parametrize("itype", [torch.int32, torch.int64])
+def test_dvals_row_ordered(orc, dtype, itype):
+    """Row-ordered dvals (column indices copied into LDS, lane groups per
+    row, dy x staged in LDS and stored in entry order, in groups of four
+    when dvals is aligned) give the same bits as the entry-ordered kernel
+    (the same one product per entry), with alpha and beta, and match the
+    oracle; stencils, bands, rows of up to 120 entries, and forced on rows
+    longer than the window and on blocks of more rows than the LDS holds."""
+    sdt = np.float64 if dtype == torch.float64 else np.float32
+    ddt = torch.float64 if dtype == torch.float64 else torch.float32
+    for name, rp, ci, v, shape, _ in _fused_rows_cases():
+        m, n = shape
+        A = drx.from_host(rp, ci, v.astype(sdt), shape, itype=itype)
+        x = drx.gen.uniform(n, ddt, -1, 1, seed=71).to(dtype)
+        dy = drx.gen.uniform(m, ddt, -1, 1, seed=72).to(dtype)
+        d0 = drx.gen.uniform(A.nnz, ddt, -1, 1, seed=73)
+        outs = {}
+        for rg in (3, 1, 0):
+            for off in (0, 1):
+                buf = torch.zeros(A.nnz + off, dtype=ddt, device=DEV)
+                o = buf[off:]
+                o.copy_(d0)
+                with drx.tuned(rowgather=rg):
+                    A.dvals(x, dy, out=o, alpha=1.5, beta=-0.5)
+                outs[(rg, off)] = o
+        torch.cuda.synchronize()
+        for key, o in outs.items():
+            assert torch.equal(o, outs[(0, 0)]), (name, key)
+        ref = 1.5 * orc.dvals(rp, ci, x.double().cpu().numpy(), dy.double().cpu().numpy()) \
+            - 0.5 * d0.double().cpu().numpy()
+        tol = 1e-15 if dtype == torch.float64 else 1e-6
+        assert np.abs(outs[(3, 0)].double().cpu().numpy() - ref).max() <= tol * 4, name
+        A.close()
