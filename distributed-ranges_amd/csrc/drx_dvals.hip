@@ -625,19 +625,20 @@ __global__ __launch_bounds__(kWG) void k_fused_residual(FusedArgs a) {
 // Row-ordered fused pass (shares of rows of at most 128 entries, the window
 // copied straight into LDS as in k_spmv_rows): G lanes per row sum it in
 // entry order (the forward kernel's order, so y equals its bits), the
-// group's lane 0 writes y and r = y - t, and each lane writes the dvals of
-// its own entries, r * x, from the x values it gathered (rows longer than
-// one round gather x again for the later rounds).
+// group's lane 0 writes y and r = y - t, and each lane puts r x of its own
+// entries in LDS at their slots from the x values it gathered (rows longer
+// than one round gather x again for the later rounds); the block's dvals
+// are then stored in entry order.
 // ---------------------------------------------------------------------------
 template <int WG, int G, int U, typename TV, typename TX, typename TY, typename TA, typename TI>
-__device__ __forceinline__ void fused_row_pass(const TV *v_s, const TI *c_s, const int *rp_s,
-                                               bool rows_lds, const FusedArgs &a, int64_t r0,
-                                               int64_t e0, int64_t nr) {
+__device__ __forceinline__ void fused_row_pass(const TV *v_s, const TI *c_s, TA *o_s,
+                                               const int *rp_s, bool rows_lds,
+                                               const FusedArgs &a, int64_t r0, int64_t e0,
+                                               int64_t nr) {
   const TI *rowptr = static_cast<const TI *>(a.rowptr);
   const TY *t = static_cast<const TY *>(a.t);
   TY *y = static_cast<TY *>(a.y);
   TY *res = static_cast<TY *>(a.r);
-  TY *dv = static_cast<TY *>(a.dvals);
   const rsrc_t rx = make_rsrc(a.x, a.xbytes);
   const int lane = threadIdx.x & (G - 1);
   for (int64_t i = threadIdx.x / G; i < nr; i += WG / G) {
@@ -692,7 +693,7 @@ __device__ __forceinline__ void fused_row_pass(const TV *v_s, const TI *c_s, con
     if (q0 < e) {
 #pragma unroll
       for (int u = 0; u < U; u++)
-        if (q0 + u * G < e) dv[e0 + q0 + u * G] = from_acc<TY>(rr * x0[u]);
+        if (q0 + u * G < e) o_s[q0 + u * G] = rr * x0[u];
     }
     for (int q = q0 + G * U; q < e; q += G * U) {
       uint32_t cc[U];
@@ -703,7 +704,7 @@ __device__ __forceinline__ void fused_row_pass(const TV *v_s, const TI *c_s, con
       for (int u = 0; u < U; u++) xx[u] = to_acc<TA>(buf_load<TX>(rx, cc[u] * sizeof(TX)));
 #pragma unroll
       for (int u = 0; u < U; u++)
-        if (q + u * G < e) dv[e0 + q + u * G] = from_acc<TY>(rr * xx[u]);
+        if (q + u * G < e) o_s[q + u * G] = rr * xx[u];
     }
   }
 }
@@ -712,6 +713,7 @@ template <int WG, int CAP, typename TV, typename TI, typename TA> struct FusedRo
   static constexpr int ROWCAP = CAP / 4;
   alignas(16) TV v[CAP];
   alignas(16) TI c[CAP];
+  alignas(16) TA o[CAP];  // r x of every entry, stored in entry order
   int rp_s[ROWCAP + 1];
   TA red[WG / 64];
 };
@@ -761,14 +763,43 @@ void k_fused_rows(FusedArgs a) {
     if (nr > 0) {
       const TV *vs = sm.v + shv;
       const TI *cs = sm.c + shc;
+      TA *os = sm.o;
       const int64_t avg = n_s / nr;
       if (avg <= 12)
-        fused_row_pass<WG, 1, 8, TV, TX, TY, TA, TI>(vs, cs, sm.rp_s, rows_lds, a, b.r0, b.e0, nr);
+        fused_row_pass<WG, 1, 8, TV, TX, TY, TA, TI>(vs, cs, os, sm.rp_s, rows_lds, a, b.r0, b.e0,
+                                                     nr);
       else if (avg <= 40)
-        fused_row_pass<WG, 8, 5, TV, TX, TY, TA, TI>(vs, cs, sm.rp_s, rows_lds, a, b.r0, b.e0, nr);
+        fused_row_pass<WG, 8, 5, TV, TX, TY, TA, TI>(vs, cs, os, sm.rp_s, rows_lds, a, b.r0, b.e0,
+                                                     nr);
       else
-        fused_row_pass<WG, 16, 5, TV, TX, TY, TA, TI>(vs, cs, sm.rp_s, rows_lds, a, b.r0, b.e0,
-                                                      nr);
+        fused_row_pass<WG, 16, 5, TV, TX, TY, TA, TI>(vs, cs, os, sm.rp_s, rows_lds, a, b.r0,
+                                                      b.e0, nr);
+    }
+    __syncthreads();
+    // dvals of the block's rows in entry order: groups of four (16-byte
+    // stores) when they are 4-byte values aligned to 16 bytes
+    {
+      TY *dv = static_cast<TY *>(a.dvals);
+      if (sizeof(TY) == 4 && reinterpret_cast<uintptr_t>(dv) % 16 == 0) {
+        const int sh4 = static_cast<int>(b.e0 & 3);
+        const int ng = (n_s + sh4 + 3) / 4;
+        for (int g = tid; g < ng; g += WG) {
+          const int q0 = g * 4 - sh4;
+          if (q0 >= 0 && q0 + 4 <= n_s) {
+            Words<4> o;
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+              o.w[i] = __builtin_bit_cast(uint32_t, static_cast<float>(from_acc<TY>(sm.o[q0 + i])));
+            *reinterpret_cast<Words<4> *>(dv + b.e0 + q0) = o;
+          } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+              if (q0 + i >= 0 && q0 + i < n_s) dv[b.e0 + q0 + i] = from_acc<TY>(sm.o[q0 + i]);
+          }
+        }
+      } else {
+        for (int q = tid; q < n_s; q += WG) dv[b.e0 + q] = from_acc<TY>(sm.o[q]);
+      }
     }
     if (b.es < e1) {
       // Long last row: the whole workgroup (the sum order of the forward
@@ -984,16 +1015,21 @@ void run_fused(const Plan &p, int vtype, const void *colind, const void *vals,
   const dim3 grid(static_cast<unsigned>(a.grid)), block(kWG);
   const bool i32 = p.itype == DRX_I32;
   const bool small = p.cap == 1024;
-  // Row-ordered pass with the window copied straight into LDS only when
-  // knob rowgather is 3: it measured slower than the entry-ordered kernel
-  // (band +-16 FP32 374 vs 353 us per call, the same box class), its
-  // row-ordered dvals stores reaching memory in 32-byte pieces.
+  // Row-ordered pass with the window copied straight into LDS for the
+  // shares the forward product sums in row order (knob rowgather: 1 by the
+  // plan's rows, 3 always, 0 and 2 never). Measured against the
+  // entry-ordered kernel (us per call, one box): band +-16 FP32 337-366 ->
+  // 313-318, FP64 588 -> 577; 3-D Laplacian 256^3 FP32 379 -> 331, FP64 654
+  // -> 603. (Storing each lane's dvals straight from the row pass instead of
+  // staging them for entry-ordered stores measured slower than the
+  // entry-ordered kernel: band FP32 374 vs 353 us.)
   {
     const int rg = tuned_rowgather();
     const uint64_t xb = static_cast<uint64_t>(std::max<int64_t>(p.ncols, 1)) * dtype_size(xtype);
     const uintptr_t vw = dtype_size(vtype) >= 4 ? 16 : 4;
     const bool rows = p.nnz > 0 && !a.xstage && p.cap == 2048 && xb < (uint64_t(1) << 32) &&
-                      rg == 3 &&
+                      (rg == 3 || (rg == 1 && p.lmax > 0 && p.lmax <= 128 &&
+                                   p.nnz >= 4 * p.nrows)) &&
                       reinterpret_cast<uintptr_t>(colind) % 16 == 0 &&
                       reinterpret_cast<uintptr_t>(vals) % vw == 0;
     if (rows) {

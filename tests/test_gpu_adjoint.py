@@ -369,16 +369,16 @@ def _fused_rows_cases():
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32, torch.bfloat16])
 @pytest.mark.parametrize("itype", [torch.int32, torch.int64])
 def test_fused_residual_row_ordered(orc, dtype, itype):
-    """The row-ordered fused kernel (rowgather 3: window copied into LDS,
-    lane groups per row, each lane writing the dvals of its own entries): y
-    equals the forward product's bits under the same knob (the same
-    row-ordered sums), r = y - t exactly, dvals within the oracle's bound.
-    The default keeps the entry-ordered fused kernel (same bits as
-    rowgather 0), also run on these shares (nnz not a multiple of four:
+    """The row-ordered fused kernel (window copied into LDS, lane groups per
+    row, r x staged in LDS and stored in entry order): y equals the forward
+    product's bits under the same knob (the same row-ordered sums), r = y -
+    t exactly, dvals within the oracle's bound. The default picks it exactly
+    for the eligible shares, the entry-ordered kernel (rowgather 0) for the
+    others; that kernel runs on every share too (nnz not a multiple of four:
     groups of four read indices past the arrays' end)."""
     low = dtype == torch.bfloat16
     sdt = np.float64 if dtype == torch.float64 else np.float32
-    for name, rp, ci, v, shape, _ in _fused_rows_cases():
+    for name, rp, ci, v, shape, eligible in _fused_rows_cases():
         m, n = shape
         A = drx.from_host(rp, ci, v.astype(sdt), shape, itype=itype)
         if low:
@@ -396,7 +396,7 @@ def test_fused_residual_row_ordered(orc, dtype, itype):
         y, r, dv = res[3]
         assert torch.equal(y, yf), name
         assert torch.equal(r, y - t), name
-        for a_, b_ in zip(res[1], res[0]):
+        for a_, b_ in zip(res[1], res[3] if eligible else res[0]):
             assert torch.equal(a_, b_), name
         y0, r0, dv0 = res[0]
         assert torch.equal(r0, y0 - t), name
