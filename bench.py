@@ -482,6 +482,23 @@ def stream_ceiling(steps=10):
                      % steps}
 
 
+def copy_ceiling(steps=10):
+    """Read + write ceiling of this box: the runtime's device-to-device copy
+    (torch copy_) of 1 GiB, counting the bytes read and the bytes written --
+    the traffic mix of dvals and the fused pass (about as many bytes written
+    as read)."""
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    src.fill_(1.0)
+    dst = torch.empty_like(src)
+    _, ms = time_launches(lambda: dst.copy_(src), steps, 2)
+    nbytes = 2 * src.numel() * 4
+    del src, dst
+    torch.cuda.empty_cache()
+    return {"GBps": round(nbytes / ms / 1e6, 1), "frac_of_peak": round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4),
+            "probe": "torch copy_ of 1 GiB device to device (1 GiB read + 1 GiB written), mean of "
+                     "%d launches" % steps}
+
+
 def form_mp_host(reps=20):
     """The reference benchmark's call shape, gemv(0, res, m, b) on a csr_eq
     distributed_sparse_matrix (benchmarks/gbench/mp/gemv.cpp:41-64), timed
@@ -581,6 +598,7 @@ def main():
     }
     log("stream ceiling ...")
     result["roofline"]["stream_ceiling"] = stream_ceiling()
+    result["roofline"]["copy_ceiling"] = copy_ceiling()
     if not args.no_cpu:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(A, x, y, args.cpu_seconds)
