@@ -246,7 +246,10 @@ __device__ __forceinline__ void row_group_sums(const VS *v_s, const CS *c_s, con
 }
 
 // Row-ordered sums of a block by its mean row length: one lane per row up
-// to 12 entries, else groups of 8 or 16 lanes.
+// to 12 entries, else groups of 8 or 16 lanes. (One lane per row with eight
+// entries per round measured faster than two rows of four per lane and one
+// row of four: 3-D Laplacian 256^3 BF16 174 vs 184 / 188 us, FP32 212 vs
+// 219 / 221 us.)
 template <int WG, typename TX, typename TY, typename TA, typename TI, typename VS, typename CS>
 __device__ __forceinline__ void row_sums_any(int n_s, const VS *v_s, const CS *c_s,
                                              const int *rp_s, bool rows_lds, const FwdArgs &a,
@@ -500,18 +503,16 @@ __device__ __forceinline__ void process_block(const FwdArgs &a, int64_t blk,
     // equal the gathered ones bit for bit.
     bool staged = false;
     if constexpr (RG) {
-      // Row-ordered gathers (k = 1; the host picks these kernels for shares
-      // of short rows): the window's values and columns go to LDS instead
-      // of products.
+      // Row-ordered gathers through registers (k = 1; the host picks these
+      // kernels for shares of rows of at most 128 entries whose vals or
+      // colind are not aligned for k_spmv_rows' straight LDS copies): the
+      // window's values and columns go to LDS instead of products.
 #pragma unroll
       for (int u = 0; u < T; u++) {
         prod[slot(u)] = v[u];
         sm.cs[slot(u)] = c[u];
       }
       __syncthreads();
-      // eight entries of a row per round (rows of 7 in one; measured against
-      // two rows of four per lane and one row of four: 3-D Laplacian 256^3
-      // BF16 174 vs 184 / 188 us, FP32 212 vs 219 / 221 us)
       row_sums_any<WG, TX, TY, TA, TI>(n_s, prod + sh, sm.cs + sh, rp_s, rows_lds, a, b.r0, b.e0,
                                        nr, y, alpha, beta);
       __syncthreads();
