@@ -860,3 +860,26 @@ def test_row_ordered_gathers(orc, vdt, itype):
             got = y.double().cpu().numpy()
             assert (np.abs(got - expect) <= tol * den + 1e-300).all(), (name, key)
         Ad.close()
+
+
+@pytest.mark.parametrize("vdt", [torch.float64, torch.float32, torch.bfloat16])
+def test_row_ordered_gathers_1024_windows(orc, vdt):
+    """Row-ordered gathers on a plan of 512-entry blocks (1024-entry windows,
+    `k_spmv_rows<256, 1024>`), default and forced, against the oracle."""
+    sdt = np.float64 if vdt == torch.float64 else np.float32
+    tol = 1e-12 if vdt == torch.float64 else 1e-5
+    for name, rp, ci, v, shape, _ in _row_gather_cases()[:4]:
+        m, n = shape
+        Ad = upload(rp, ci, v.astype(sdt), shape, block_nnz=512)
+        vq = Ad.vals.to(vdt)
+        xdt = torch.float64 if vdt == torch.float64 else torch.float32
+        xq = drx.gen.uniform(n, xdt, -1, 1, seed=31).to(vdt)
+        ys = {}
+        for rg in (1, 3):
+            with drx.tuned(rowgather=rg):
+                ys[rg] = Ad.matvec(xq, vals=vq)
+        torch.cuda.synchronize()
+        ystar, absy = orc.spmv(rp, ci, vq.double().cpu().numpy(), xq.double().cpu().numpy())
+        for rg, y in ys.items():
+            assert componentwise_err(y.double().cpu().numpy(), ystar, absy) < tol, (name, rg)
+        Ad.close()
