@@ -20,6 +20,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "drx_common.hpp"
 #include "drx_internal.hpp"
 
@@ -159,8 +161,11 @@ __device__ __forceinline__ void reduce_rows_any(int n_s, const TA *prod,
 // the same entries span every neighbour's lines. Slots past a row's end
 // repeat its first entry (in range, and its x is gathered anyway) and add
 // nothing.
+// PERM (the adjoint through a permutation): v_s holds each entry's position
+// in vals, and the values are gathered too (through a buffer resource over
+// the share's nnz values of type TV).
 template <int WG, int U, typename TX, typename TY, typename TA, typename TI,
-          typename VS = TA, typename CS = uint32_t>
+          typename VS = TA, typename CS = uint32_t, bool PERM = false, typename TV = TA>
 __device__ __forceinline__ void row_gather_sums(const VS *v_s, const CS *c_s,
                                                 const int *rp_s, bool rows_lds,
                                                 const FwdArgs &a, int64_t r0,
@@ -168,6 +173,8 @@ __device__ __forceinline__ void row_gather_sums(const VS *v_s, const CS *c_s,
                                                 TA alpha, TA beta) {
   const TI *rowptr = static_cast<const TI *>(a.rowptr);
   const rsrc_t rx = make_rsrc(a.x, a.xbytes);
+  [[maybe_unused]] const rsrc_t rv =
+      make_rsrc(a.vals, PERM ? static_cast<uint32_t>(a.nnz * sizeof(TV)) : 0u);
   for (int64_t i = threadIdx.x; i < nr; i += WG) {
     int s, e;
     if (rows_lds) {
@@ -185,7 +192,10 @@ __device__ __forceinline__ void row_gather_sums(const VS *v_s, const CS *c_s,
       for (int u = 0; u < U; u++) {
         const int qq = q + u < e ? q + u : q;
         cc[u] = col32(c_s[qq]);
-        vv[u] = to_acc<TA>(v_s[qq]);
+        if constexpr (PERM)
+          vv[u] = to_acc<TA>(buf_load<TV>(rv, col32(v_s[qq]) * static_cast<uint32_t>(sizeof(TV))));
+        else
+          vv[u] = to_acc<TA>(v_s[qq]);
       }
 #pragma unroll
       for (int u = 0; u < U; u++) xx[u] = to_acc<TA>(buf_load<TX>(rx, cc[u] * sizeof(TX)));
@@ -203,13 +213,15 @@ __device__ __forceinline__ void row_gather_sums(const VS *v_s, const CS *c_s,
 // takes entries j, j + G, ..., U of them per round with their gathers in
 // flight, and the group's sums are added with DPP / shuffles.
 template <int WG, int G, int U, typename TX, typename TY, typename TA, typename TI,
-          typename VS, typename CS>
+          typename VS, typename CS, bool PERM = false, typename TV = TA>
 __device__ __forceinline__ void row_group_sums(const VS *v_s, const CS *c_s, const int *rp_s,
                                                bool rows_lds, const FwdArgs &a, int64_t r0,
                                                int64_t e0, int64_t nr, TY *y, TA alpha,
                                                TA beta) {
   const TI *rowptr = static_cast<const TI *>(a.rowptr);
   const rsrc_t rx = make_rsrc(a.x, a.xbytes);
+  [[maybe_unused]] const rsrc_t rv =
+      make_rsrc(a.vals, PERM ? static_cast<uint32_t>(a.nnz * sizeof(TV)) : 0u);
   const int lane = threadIdx.x & (G - 1);
   for (int64_t i = threadIdx.x / G; i < nr; i += WG / G) {
     int s, e;
@@ -228,7 +240,10 @@ __device__ __forceinline__ void row_group_sums(const VS *v_s, const CS *c_s, con
       for (int u = 0; u < U; u++) {
         const int qq = q + u * G < e ? q + u * G : q;
         cc[u] = col32(c_s[qq]);
-        vv[u] = to_acc<TA>(v_s[qq]);
+        if constexpr (PERM)
+          vv[u] = to_acc<TA>(buf_load<TV>(rv, col32(v_s[qq]) * static_cast<uint32_t>(sizeof(TV))));
+        else
+          vv[u] = to_acc<TA>(v_s[qq]);
       }
 #pragma unroll
       for (int u = 0; u < U; u++) xx[u] = to_acc<TA>(buf_load<TX>(rx, cc[u] * sizeof(TX)));
@@ -250,7 +265,8 @@ __device__ __forceinline__ void row_group_sums(const VS *v_s, const CS *c_s, con
 // entries per round measured faster than two rows of four per lane and one
 // row of four: 3-D Laplacian 256^3 BF16 174 vs 184 / 188 us, FP32 212 vs
 // 219 / 221 us.)
-template <int WG, typename TX, typename TY, typename TA, typename TI, typename VS, typename CS>
+template <int WG, typename TX, typename TY, typename TA, typename TI, typename VS, typename CS,
+          bool PERM = false, typename TV = TA>
 __device__ __forceinline__ void row_sums_any(int n_s, const VS *v_s, const CS *c_s,
                                              const int *rp_s, bool rows_lds, const FwdArgs &a,
                                              int64_t r0, int64_t e0, int64_t nr, TY *y, TA alpha,
@@ -259,21 +275,24 @@ __device__ __forceinline__ void row_sums_any(int n_s, const VS *v_s, const CS *c
   // one lane per row: rounds of the share's longest row when that is 5 or
   // 7 (2-D and 3-D stencils), so that no gather slot of a round is idle;
   // the sums are the same either way (entry order)
+#define DRX_RG(U)                                                                           \
+  row_gather_sums<WG, U, TX, TY, TA, TI, VS, CS, PERM, TV>(v_s, c_s, rp_s, rows_lds, a, r0, e0, \
+                                                           nr, y, alpha, beta)
+#define DRX_RGG(G)                                                                          \
+  row_group_sums<WG, G, 5, TX, TY, TA, TI, VS, CS, PERM, TV>(v_s, c_s, rp_s, rows_lds, a, r0,  \
+                                                             e0, nr, y, alpha, beta)
   if (avg <= 12 && a.lmax > 0 && a.lmax <= 5)
-    row_gather_sums<WG, 5, TX, TY, TA, TI, VS, CS>(v_s, c_s, rp_s, rows_lds, a, r0, e0, nr, y,
-                                                   alpha, beta);
+    DRX_RG(5);
   else if (avg <= 12 && a.lmax > 5 && a.lmax <= 7)
-    row_gather_sums<WG, 7, TX, TY, TA, TI, VS, CS>(v_s, c_s, rp_s, rows_lds, a, r0, e0, nr, y,
-                                                   alpha, beta);
+    DRX_RG(7);
   else if (avg <= 12)
-    row_gather_sums<WG, 8, TX, TY, TA, TI, VS, CS>(v_s, c_s, rp_s, rows_lds, a, r0, e0, nr, y,
-                                                   alpha, beta);
+    DRX_RG(8);
   else if (avg <= 40)
-    row_group_sums<WG, 8, 5, TX, TY, TA, TI, VS, CS>(v_s, c_s, rp_s, rows_lds, a, r0, e0, nr, y,
-                                                     alpha, beta);
+    DRX_RGG(8);
   else
-    row_group_sums<WG, 16, 5, TX, TY, TA, TI, VS, CS>(v_s, c_s, rp_s, rows_lds, a, r0, e0, nr, y,
-                                                      alpha, beta);
+    DRX_RGG(16);
+#undef DRX_RGG
+#undef DRX_RG
 }
 
 template <int MODE, typename TV, typename TI>
@@ -786,11 +805,13 @@ __device__ __forceinline__ void rows_offsets_store(const Args &a, int64_t r0, in
 // A last row [es, e1) longer than the window (row r1 - 1): the whole
 // workgroup, strided coalesced loads, in the sum order of process_block's
 // long-row path (the same bits).
-template <int WG, typename TV, typename TX, typename TY, typename TA, typename TI>
+template <int WG, typename TV, typename TX, typename TY, typename TA, typename TI,
+          int MODE = kPlain>
 __device__ __forceinline__ void rows_long_last(const FwdArgs &a, int64_t es, int64_t e1,
                                                int64_t r1, TA *red, int tid) {
   const TI *colind = static_cast<const TI *>(a.colind);
   const TV *vals = static_cast<const TV *>(a.vals);
+  const TI *perm = static_cast<const TI *>(a.perm);
   const TX *x = static_cast<const TX *>(a.x);
   TY *y = static_cast<TY *>(a.y);
   const TA alpha = static_cast<TA>(a.alpha), beta = static_cast<TA>(a.beta);
@@ -798,8 +819,9 @@ __device__ __forceinline__ void rows_long_last(const FwdArgs &a, int64_t es, int
   TA acc0 = 0, acc1 = 0;
   int64_t q = es + tid;
   for (; q + 3 * WG < e1; q += 4 * WG) {
-    const TV w0 = ld_stream(vals + q), w1 = ld_stream(vals + q + WG);
-    const TV w2 = ld_stream(vals + q + 2 * WG), w3 = ld_stream(vals + q + 3 * WG);
+    const TV w0 = long_val<MODE>(vals, perm, q), w1 = long_val<MODE>(vals, perm, q + WG);
+    const TV w2 = long_val<MODE>(vals, perm, q + 2 * WG);
+    const TV w3 = long_val<MODE>(vals, perm, q + 3 * WG);
     const TI c0 = ld_stream(colind + q), c1 = ld_stream(colind + q + WG);
     const TI c2 = ld_stream(colind + q + 2 * WG), c3 = ld_stream(colind + q + 3 * WG);
     const TA x0 = to_acc<TA>(x[static_cast<int64_t>(c0)]);
@@ -812,7 +834,7 @@ __device__ __forceinline__ void rows_long_last(const FwdArgs &a, int64_t es, int
     acc1 += to_acc<TA>(w3) * x3;
   }
   for (; q < e1; q += WG)
-    acc0 += to_acc<TA>(ld_stream(vals + q)) *
+    acc0 += to_acc<TA>(long_val<MODE>(vals, perm, q)) *
             to_acc<TA>(x[static_cast<int64_t>(ld_stream(colind + q))]);
   const TA s = block_sum<WG>(acc0 + acc1, red);
   if (tid == 0) {
@@ -822,13 +844,23 @@ __device__ __forceinline__ void rows_long_last(const FwdArgs &a, int64_t es, int
   }
 }
 
-template <int WG, int CAP, typename TV, typename TX, typename TY, typename TA, typename TI>
-__device__ __forceinline__ void process_block_rows(const FwdArgs &a, int64_t blk,
-                                                   RowsSmem<WG, CAP, TV, TI, TA> &sm, int tid) {
-  constexpr int ROWCAP = RowsSmem<WG, CAP, TV, TI, TA>::ROWCAP;
+// MODE kPerm (the adjoint through a permutation): the window's positions in
+// vals (perm) are copied into LDS in place of the values, and each entry
+// gathers its value: a wave's value gathers then read the u-th entries of
+// 64 consecutive rows (of the transposed structure), which for stencils and
+// bands sit next to each other in vals.
+template <int MODE, typename TV, typename TI>
+using RowsVal = std::conditional_t<MODE == kPerm, TI, TV>;
+
+template <int WG, int CAP, typename TV, typename TX, typename TY, typename TA, typename TI,
+          int MODE = kPlain>
+__device__ __forceinline__ void process_block_rows(
+    const FwdArgs &a, int64_t blk, RowsSmem<WG, CAP, RowsVal<MODE, TV, TI>, TI, TA> &sm, int tid) {
+  using VS = RowsVal<MODE, TV, TI>;
+  constexpr int ROWCAP = RowsSmem<WG, CAP, VS, TI, TA>::ROWCAP;
   constexpr int NI = (ROWCAP + WG) / WG;
   const TI *colind = static_cast<const TI *>(a.colind);
-  const TV *vals = static_cast<const TV *>(a.vals);
+  const VS *vsrc = static_cast<const VS *>(MODE == kPerm ? a.perm : a.vals);
   TY *y = static_cast<TY *>(a.y);
   const TA alpha = static_cast<TA>(a.alpha), beta = static_cast<TA>(a.beta);
   const BlockRec b = a.rec[blk];
@@ -837,7 +869,7 @@ __device__ __forceinline__ void process_block_rows(const FwdArgs &a, int64_t blk
   const int n_s = static_cast<int>(b.es - b.e0);
   const int64_t nr = (b.es == e1 ? r1 : r1 - 1) - b.r0;
   const bool rows_lds = nr < ROWCAP;
-  const int shv = lds_window<WG, CAP>(vals, b.e0, n_s, sm.v, tid);
+  const int shv = lds_window<WG, CAP>(vsrc, b.e0, n_s, sm.v, tid);
   const int shc = lds_window<WG, CAP>(colind, b.e0, n_s, sm.c, tid);
   if (rows_lds) {
     TI raw[NI];
@@ -847,9 +879,9 @@ __device__ __forceinline__ void process_block_rows(const FwdArgs &a, int64_t blk
   // the LDS copies count as vector-memory loads
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
-  row_sums_any<WG, TX, TY, TA, TI>(n_s, sm.v + shv, sm.c + shc, sm.rp_s, rows_lds, a, b.r0, b.e0,
-                                   nr, y, alpha, beta);
-  if (b.es < e1) rows_long_last<WG, TV, TX, TY, TA, TI>(a, b.es, e1, r1, sm.red, tid);
+  row_sums_any<WG, TX, TY, TA, TI, VS, TI, MODE == kPerm, TV>(
+      n_s, sm.v + shv, sm.c + shc, sm.rp_s, rows_lds, a, b.r0, b.e0, nr, y, alpha, beta);
+  if (b.es < e1) rows_long_last<WG, TV, TX, TY, TA, TI, MODE>(a, b.es, e1, r1, sm.red, tid);
   __syncthreads();  // the next block overwrites the LDS window
 }
 
@@ -862,17 +894,18 @@ template <int WG, int CAP, typename TV, typename TI, typename TA> constexpr int 
   return w > most ? most : (w < 1 ? 1 : w);
 }
 
-template <int WG, int CAP, typename TV, typename TX, typename TY, typename TA, typename TI>
+template <int WG, int CAP, typename TV, typename TX, typename TY, typename TA, typename TI,
+          int MODE = kPlain>
 __global__ __launch_bounds__(WG)
-__attribute__((amdgpu_waves_per_eu(rows_waves<WG, CAP, TV, TI, TA>(), 8)))
+__attribute__((amdgpu_waves_per_eu(rows_waves<WG, CAP, RowsVal<MODE, TV, TI>, TI, TA>(), 8)))
 void k_spmv_rows(FwdArgs a) {
-  __shared__ RowsSmem<WG, CAP, TV, TI, TA> sm;
+  __shared__ RowsSmem<WG, CAP, RowsVal<MODE, TV, TI>, TI, TA> sm;
   int tid;
   asm volatile("" : "=v"(tid) : "0"(static_cast<int>(threadIdx.x)));
   for (int64_t wi = blockIdx.x; wi < a.per_xcd * kXcds; wi += a.grid) {
     const int64_t blk = map_block(wi, a.per_xcd);
     if (blk >= a.nblocks) continue;
-    process_block_rows<WG, CAP, TV, TX, TY, TA, TI>(a, blk, sm, tid);
+    process_block_rows<WG, CAP, TV, TX, TY, TA, TI, MODE>(a, blk, sm, tid);
   }
 }
 

@@ -162,6 +162,17 @@ static void launch_fwd(const Plan &p, FwdArgs a, hipStream_t s) {
       cap = want;
     }
   }
+  if constexpr (MODE == kPerm && BUFX) {
+    // the adjoint through a permutation: row-ordered, perm copied into LDS
+    // (perm and colind aligned to 16 bytes)
+    if (a.rowgather && a.rowgather != 2 && a.k == 1 && cap == 2048 &&
+        reinterpret_cast<uintptr_t>(a.perm) % 16 == 0 &&
+        reinterpret_cast<uintptr_t>(a.colind) % 16 == 0) {
+      hipLaunchKernelGGL((k_spmv_rows<256, 2048, TV, TX, TY, TA, TI, kPerm>), dim3(a.grid),
+                         dim3(256), 0, s, a);
+      return;
+    }
+  }
   if constexpr (MODE == kPlain && BUFX) {
     if (a.rowgather && a.k == 1) {
       // The window copied straight into LDS (vals aligned to its copy unit,
@@ -308,6 +319,17 @@ static void run_spmv_impl(const Plan &p, int vtype, const void *colind,
   // Staging each block's x window in LDS (one right-hand side) only when
   // the knob forces it: with all of a lane's gathers in flight, per-entry
   // gathers beat it on bands (+-16 FP32: 241 vs 286 us per call).
+  if constexpr (MODE == kPerm) {
+    // Row-ordered gathers of x and of the values through the permutation
+    // (values gathered through a buffer resource: nnz values within 4 GB).
+    if (k == 1 && bufx && p.nnz > 0 &&
+        static_cast<uint64_t>(p.nnz) * dtype_size(vtype) < (uint64_t(1) << 32)) {
+      const int rg = tuned_rowgather();
+      a.rowgather = rg >= 2 ? rg
+                    : (rg == 1 && p.lmax > 0 && p.lmax <= 128 && p.nnz >= 4 * p.nrows) ? 1
+                                                                                     : 0;
+    }
+  }
   if constexpr (MODE == kPlain) {
     if (k == 1 && bufx && p.nnz > 0) a.xstage = tuned_xstage() == 2;
     // Row-ordered gathers for shares of rows of at most 128 entries (at

@@ -125,10 +125,11 @@ const char *drx_last_error(void);
  *                 into LDS where vals and colind are aligned to 16 bytes (4
  *                 for 16-bit values), else through registers; 2 always,
  *                 through registers; 3 always, straight copies where
- *                 aligned; 0 never. drx_csr_dvals (one right-hand side)
- *                 and drx_spmv_fused_residual follow the same choice with
- *                 row-ordered kernels of their own (0 and 2: their
- *                 entry-ordered kernels). Sums in entry order, so results may
+ *                 aligned; 0 never. The adjoint through the permutation
+ *                 (drx_spmv_adjoint without stored CSC values), drx_csr_dvals
+ *                 (one right-hand side) and drx_spmv_fused_residual follow
+ *                 the same choice with row-ordered kernels of their own (0
+ *                 and 2: their entry-ordered kernels). Sums in entry order, so results may
  *                 differ from 0 in the last bits (all within the oracle
  *                 tolerance).
  *   "gemm_tile"   drx_gemm workgroup tile: 1 128 x 128; 2 256 x 128; 3 and

@@ -253,23 +253,30 @@ def test_autograd_gradcheck_small():
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_adjoint_with_stored_transposed_values(orc, dtype):
     """Values stored in CSC order give the same adjoint, bit for bit, as
-    reading them through the permutation (on the same 1024-entry blocks and
-    with entry-ordered gathers: by default the stored-values path streams
-    2048-entry blocks for FP32 and sums short rows in row order, checked
-    against the oracle); changing the values in place (new tensor version)
-    falls back to the permutation."""
+    reading them through the permutation: by default both sum the short rows
+    of the transposed structure in row order; with entry-ordered gathers
+    (rowgather 0) on the same 1024-entry blocks (the stored-values FP32
+    path otherwise streams 2048-entry blocks, checked against the oracle).
+    Changing the values in place (new tensor version) falls back to the
+    permutation."""
     A = rand_csr(5000, 4000, 0.002, 21)
     v = A.data.astype(np.float64 if dtype == torch.float64 else np.float32)
     Ad = drx.from_host(A.indptr, A.indices, v, A.shape)
     dy = torch.as_tensor(np.random.default_rng(2).uniform(-1, 1, A.shape[0]).astype(v.dtype)).to(DEV)
     via_perm = Ad.rmatvec(dy)
+    with drx.tuned(rowgather=0):
+        via_perm0 = Ad.rmatvec(dy)
     Ad.set_transposed_values()
     assert Ad.stored_transposed_values(Ad.vals)
     with drx.tuned(wide=0, rowgather=0):
-        stored = Ad.rmatvec(dy)
+        stored0 = Ad.rmatvec(dy)
     stored_wide = Ad.rmatvec(dy)
     torch.cuda.synchronize()
-    assert torch.equal(stored, via_perm)
+    assert torch.equal(stored0, via_perm0)
+    assert torch.equal(stored_wide, via_perm)
+    ref_p, absr_p = orc.spmv_t(A.indptr, A.indices, v, dy.cpu().numpy(), A.shape[1])
+    tol_p = 1e-12 if dtype == torch.float64 else 1e-5
+    assert componentwise_err(via_perm.cpu().numpy(), ref_p, absr_p) < tol_p
     ref, absr = orc.spmv_t(A.indptr, A.indices, v, dy.cpu().numpy(), A.shape[1])
     tol = 1e-12 if dtype == torch.float64 else 1e-5
     assert componentwise_err(stored_wide.cpu().numpy(), ref, absr) < tol
