@@ -852,10 +852,20 @@ __device__ __forceinline__ void rows_long_last(const FwdArgs &a, int64_t es, int
 template <int MODE, typename TV, typename TI>
 using RowsVal = std::conditional_t<MODE == kPerm, TI, TV>;
 
+// A block's bounds: its record and the next one's first row and entry.
+struct RowsRec {
+  BlockRec b;
+  int64_t r1, e1;
+};
+__device__ __forceinline__ RowsRec rows_rec(const FwdArgs &a, int64_t blk) {
+  return RowsRec{a.rec[blk], a.rec[blk + 1].r0, a.rec[blk + 1].e0};
+}
+
 template <int WG, int CAP, typename TV, typename TX, typename TY, typename TA, typename TI,
           int MODE = kPlain>
 __device__ __forceinline__ void process_block_rows(
-    const FwdArgs &a, int64_t blk, RowsSmem<WG, CAP, RowsVal<MODE, TV, TI>, TI, TA> &sm, int tid) {
+    const FwdArgs &a, const RowsRec &rr, RowsSmem<WG, CAP, RowsVal<MODE, TV, TI>, TI, TA> &sm,
+    int tid) {
   using VS = RowsVal<MODE, TV, TI>;
   constexpr int ROWCAP = RowsSmem<WG, CAP, VS, TI, TA>::ROWCAP;
   constexpr int NI = (ROWCAP + WG) / WG;
@@ -863,8 +873,8 @@ __device__ __forceinline__ void process_block_rows(
   const VS *vsrc = static_cast<const VS *>(MODE == kPerm ? a.perm : a.vals);
   TY *y = static_cast<TY *>(a.y);
   const TA alpha = static_cast<TA>(a.alpha), beta = static_cast<TA>(a.beta);
-  const BlockRec b = a.rec[blk];
-  const int64_t r1 = a.rec[blk + 1].r0, e1 = a.rec[blk + 1].e0;
+  const BlockRec b = rr.b;
+  const int64_t r1 = rr.r1, e1 = rr.e1;
   if (b.r0 >= r1) return;
   const int n_s = static_cast<int>(b.es - b.e0);
   const int64_t nr = (b.es == e1 ? r1 : r1 - 1) - b.r0;
@@ -902,10 +912,26 @@ void k_spmv_rows(FwdArgs a) {
   __shared__ RowsSmem<WG, CAP, RowsVal<MODE, TV, TI>, TI, TA> sm;
   int tid;
   asm volatile("" : "=v"(tid) : "0"(static_cast<int>(threadIdx.x)));
-  for (int64_t wi = blockIdx.x; wi < a.per_xcd * kXcds; wi += a.grid) {
-    const int64_t blk = map_block(wi, a.per_xcd);
-    if (blk >= a.nblocks) continue;
-    process_block_rows<WG, CAP, TV, TX, TY, TA, TI, MODE>(a, blk, sm, tid);
+  // Workgroups that walk several blocks (capped grids) load the next
+  // block's record before the current block's window copies, so that its
+  // round trip overlaps theirs.
+  const int64_t total = a.per_xcd * kXcds;
+  auto next_valid = [&](int64_t w) {
+    for (; w < total; w += a.grid)
+      if (map_block(w, a.per_xcd) < a.nblocks) break;
+    return w;
+  };
+  int64_t wi = next_valid(blockIdx.x);
+  if (wi >= total) return;
+  RowsRec cur = rows_rec(a, map_block(wi, a.per_xcd));
+  for (;;) {
+    const int64_t wn = next_valid(wi + a.grid);
+    RowsRec nxt = cur;
+    if (wn < total) nxt = rows_rec(a, map_block(wn, a.per_xcd));
+    process_block_rows<WG, CAP, TV, TX, TY, TA, TI, MODE>(a, cur, sm, tid);
+    if (wn >= total) break;
+    wi = wn;
+    cur = nxt;
   }
 }
 
